@@ -1,0 +1,101 @@
+"""ctypes binding of the C ABI in include/ever_amd.h (libever_amd.so, built for gfx950).
+
+PyTorch provides device memory and streams; every compute call goes to the HIP library. There is no fallback:
+if the library is missing or fails to load, `lib()` raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads PyTorch's HIP runtime first, so the library binds to the same runtime)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libever_amd.so")
+
+EVER_OK = 0
+EVER_COLOR_DC = 0
+EVER_COLOR_SH = 1
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+
+
+class Prims(ctypes.Structure):
+    _fields_ = [("means", _vp), ("scales", _vp), ("quats", _vp), ("densities", _vp), ("features", _vp),
+                ("num_prims", _i64), ("feature_size", _i32), ("color_mode", _i32), ("sh_degree", _i32),
+                ("activation", _i32), ("sh_origin", ctypes.c_float * 3), ("reserved", ctypes.c_float)]
+
+
+class Rays(ctypes.Structure):
+    _fields_ = [("origins", _vp), ("directions", _vp), ("num_rays", _i64), ("tmin", ctypes.c_float),
+                ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32)]
+
+
+class ForwardOut(ctypes.Structure):
+    _fields_ = [("color", _vp), ("states", _vp), ("iters", _vp), ("tri_collection", _vp), ("touch_count", _vp),
+                ("initial_drgb", _vp), ("ray_state64", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t)]
+
+
+class BackwardIO(ctypes.Structure):
+    _fields_ = [("grad_out", _vp), ("iters", _vp), ("tri_collection", _vp), ("states", _vp), ("ray_state64", _vp),
+                ("wcts", _vp), ("num_wcts", _i64), ("dmeans", _vp), ("dscales", _vp), ("dquats", _vp),
+                ("ddensities", _vp), ("dfeatures", _vp), ("drayo", _vp), ("drayd", _vp), ("dmeans2D", _vp),
+                ("dinitial_drgb", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t)]
+
+
+# Every function the header declares, with its ctypes signature.
+SIGNATURES = {
+    "ever_abi_version": ([], ctypes.c_int),
+    "ever_struct_sizes": ([ctypes.POINTER(_i64)], ctypes.c_int),
+    "ever_last_error": ([], ctypes.c_char_p),
+    "ever_scene_workspace_bytes": ([_i64], ctypes.c_size_t),
+    "ever_build_scene": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "ever_forward_workspace_bytes": ([_i64], ctypes.c_size_t),
+    "ever_trace_forward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
+                            ctypes.POINTER(ForwardOut), _vp], ctypes.c_int),
+    "ever_backward_workspace_bytes": ([_i64, _i64, _i64], ctypes.c_size_t),
+    "ever_trace_backward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
+                             ctypes.POINTER(BackwardIO), _vp], ctypes.c_int),
+    "ever_eval_sh": ([_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
+    "ever_eval_sh_backward": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
+}
+
+_lib = None
+
+
+class EverError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libever_amd.so; raises if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EverError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        sizes = (_i64 * 4)()
+        L.ever_struct_sizes(sizes)
+        mine = [ctypes.sizeof(Prims), ctypes.sizeof(Rays), ctypes.sizeof(ForwardOut), ctypes.sizeof(BackwardIO)]
+        if list(sizes) != mine:
+            raise EverError(f"ABI struct size mismatch: library {list(sizes)} vs binding {mine}")
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != EVER_OK:
+        msg = lib().ever_last_error().decode()
+        raise EverError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,435 @@
+// Backward tracer: one lane per ray replays the ray's stored events in reverse, reconstructing each earlier
+// state from the FP64 accumulators written by the forward (exact inverse of the forward's FP64 additions),
+// applies the hand-derived adjoint of `update` and of the safe ellipsoid roots, and scatters per-primitive
+// gradients (mean, scale, quaternion, density, colour) into a 64-byte per-primitive accumulator with float
+// atomics. A per-primitive epilogue turns the colour gradient into SH-feature gradients, applies the
+// reference's clamps and the means2D statistic.
+//
+// Semantics restated from the reference (splinetracers/fast_ellipsoid_splinetracer/slang/backwards_kernel.slang):
+//   per-ray skip rule :303-304, bwd of extract_color :307, reverse loop :327-361 (bwd_diff(update) :154,
+//   bwd_diff(safe_intersect) :172, colour :184, means2D :213-220), dL/d initial drgb :363-366,
+//   start-inside adjoint :374-428; clamps splinetracers/fast_ellipsoid_splinetracer.py:201-211.
+#include "ever_common.hpp"
+#include "ever_internal.hpp"
+
+namespace ever {
+
+constexpr int kBwdBlock = 64;
+
+struct BwdArgs {
+  const float* means;
+  const float* scales;
+  const float* quats;
+  const float4* colr;
+  const float* rayo;
+  const float* rayd;
+  int64_t nr;
+  float tmin;
+  int max_iters;
+  const float* grad_out;
+  const int32_t* iters;
+  const int32_t* tri;
+  const float* states;
+  const double* st64;
+  const float* wcts;
+  int64_t nw;
+  const float* inv;  // (nw,16) inverses
+  float* acc;        // (N,16)
+  float* drayo;
+  float* drayd;
+  float* dinit;      // (R,4) scratch or caller buffer
+};
+
+__global__ void k_zero_f32(float* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+// Inverse of each 4x4 world-to-camera matrix (the reference inverts per ray, safe-math.slang:187-225), FP64
+// Gauss-Jordan with partial pivoting.
+__global__ void k_invert4(const float* __restrict__ w, int64_t nw, float* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nw) return;
+  double a[4][8];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 8; ++j) a[i][j] = j < 4 ? (double)w[16 * k + 4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 4; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+    for (int j = 0; j < 8; ++j) {
+      double t = a[c][j];
+      a[c][j] = a[piv][j];
+      a[piv][j] = t;
+    }
+    double p = a[c][c];
+    for (int j = 0; j < 8; ++j) a[c][j] /= p;
+    for (int r = 0; r < 4; ++r)
+      if (r != c) {
+        double f = a[r][c];
+        for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+      }
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out[16 * k + 4 * i + j] = (float)a[i][j + 4];
+}
+
+// safe root of one event, value only
+__device__ __forceinline__ float event_t(const BwdArgs& A, f3 org, f3 dir, int32_t id) {
+  const int64_t p = id >> 1;
+  SafeRoots S;
+  float4 q = reinterpret_cast<const float4*>(A.quats)[p];
+  safe_roots(org - ld3(A.means + 3 * p), dir, ld3(A.scales + 3 * p), q, S);
+  return safe_root_select(S, id & 1);
+}
+
+__global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
+  const int64_t r = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
+  if (r >= A.nr) return;
+  const int E = A.iters[r];
+  float4* dinit = reinterpret_cast<float4*>(A.dinit) + r;
+  if (E >= A.max_iters - 1 || E <= 0) {  // backwards_kernel.slang:304
+    *dinit = make_float4(0.f, 0.f, 0.f, 0.f);
+    float* o = A.drayo + 3 * r;
+    float* dd = A.drayd + 3 * r;
+    o[0] = o[1] = o[2] = 0.f;
+    dd[0] = dd[1] = dd[2] = 0.f;
+    return;
+  }
+  const f3 dir = ld3(A.rayd + 3 * r);  // raw direction (backwards_kernel.slang:288-289)
+  const f3 org = ld3(A.rayo + 3 * r) + A.tmin * dir;
+  const float* g = A.grad_out + 5 * r;
+  const float gC0 = g[0], gC1 = g[1], gC2 = g[2], gO = g[3], gD = g[4];
+  const float4* st = reinterpret_cast<const float4*>(A.states + 16 * r);
+  const float4 st0 = st[0];
+  const double* s64 = A.st64 + 6 * r;
+  // state after the current event (starts as the final state)
+  double Da0 = s64[0], Da1 = s64[1], Da2 = s64[2], Da3 = s64[3], La = s64[4];
+  float csxa = st0.z, csya = st0.w;
+  // adjoints of the state after the current event
+  const float dpxb = gD, dpyb = -gD;
+  float Lb = gO * (float)exp(-La);
+  float csxb = 0.f, csyb = 0.f, tcarry = 0.f;
+  float Db0 = 0.f, Db1 = 0.f, Db2 = 0.f, Db3 = 0.f;
+  f3 orgb = mk3(0.f, 0.f, 0.f), dirb = mk3(0.f, 0.f, 0.f);
+  const float* inv = nullptr;
+  const float* W = nullptr;
+  if (A.nw > 1) {
+    const int64_t wi = r < A.nw ? r : 0;  // backwards_kernel.slang:313
+    inv = A.inv + 16 * wi;
+    W = A.wcts + 16 * wi;
+  }
+
+  int32_t id = A.tri[r + (int64_t)(E - 1) * A.nr];
+  float tk = event_t(A, org, dir, id);
+  for (int k = E - 1; k >= 0; --k) {
+    int32_t id_prev = 0;
+    float tprev = 0.f;
+    if (k > 0) {
+      id_prev = A.tri[r + (int64_t)(k - 1) * A.nr];
+      tprev = event_t(A, org, dir, id_prev);
+    }
+    const int64_t p = id >> 1;
+    const int face = id & 1;
+    const float4 cr = A.colr[p];
+    const float hgt = face ? cr.x : -cr.x;
+    // reconstruct the state before event k (inverse of the forward's FP64 accumulation)
+    const double Dbf0 = Da0 - (double)hgt, Dbf1 = Da1 - (double)(hgt * cr.y), Dbf2 = Da2 - (double)(hgt * cr.z),
+                 Dbf3 = Da3 - (double)(hgt * cr.w);
+    const float dt_raw = tk - tprev;
+    const float dt = fmaxf(dt_raw, 0.f);
+    const float Df = (float)Dbf0;
+    const float a_raw = Df * dt;
+    const float area = fmaxf(a_raw, 0.f);
+    const double Lbf = fmax(La - (double)area, 0.0);
+    const float e1 = expf(-area), om = -expm1f(-area);
+    const float e2 = expf(-(float)Lbf);
+    const float w_raw = om * e2;
+    const float w = clipf(w_raw, 0.f, 1.f);
+    const float mid = (tk + tprev) * (0.5f * kPreMulti);
+    const float m = ladder(mid);
+    const float csxbf = csxa - w, csybf = csya - w * m;
+    const float R1 = (float)Dbf1, R2 = (float)Dbf2, R3 = (float)Dbf3;
+    const float rn0 = safe_div(R1, Df), rn1 = safe_div(R2, Df), rn2 = safe_div(R3, Df);
+    // adjoint of update (spline-machine.slang:103-138)
+    const float wb = gC0 * rn0 + gC1 * rn1 + gC2 * rn2 + dpxb * 2.f * m * csxbf + dpyb * 2.f * csybf + csxb + csyb * m;
+    const float mb = dpxb * 2.f * w * csxbf + csyb * w;
+    const float dirac0 = Db0, dirac1 = Db1, dirac2 = Db2, dirac3 = Db3;  // adjoint of the new state's drgb
+    const float csxb_n = csxb + dpxb * 2.f * w * m;
+    const float csyb_n = csyb + dpyb * 2.f * w;
+    const float midb = mb * ladder_grad(mid) * (0.5f * kPreMulti);
+    float tb = tcarry + midb;
+    float tprevb = midb;
+    const float wrawb = (w_raw < 1.f && w_raw > 0.f) ? wb : 0.f;
+    float areab = wrawb * e2 * e1;
+    const float L_raw = area + (float)Lbf;
+    float Lb_n = -(wrawb * om) * e2;
+    if (L_raw > 0.f) {
+      areab += Lb;
+      Lb_n += Lb;
+    }
+    float Dbar = 0.f, da, db;
+    bw_safe_div(R1, Df, gC0 * w, da, db);
+    const float Db1_n = Db1 + da;
+    Dbar += db;
+    bw_safe_div(R2, Df, gC1 * w, da, db);
+    const float Db2_n = Db2 + da;
+    Dbar += db;
+    bw_safe_div(R3, Df, gC2 * w, da, db);
+    const float Db3_n = Db3 + da;
+    Dbar += db;
+    const float arawb = a_raw > 0.f ? areab : 0.f;
+    Dbar += arawb * dt;
+    const float dtb = arawb * Df;
+    const float Db0_n = Db0 + Dbar;
+    if (dt_raw > 0.f) {
+      tb += dtb;
+      tprevb -= dtb;
+    }
+    // control point of event k: dirac = +-density * (1, colour), t = selected safe root
+    const float sg = face ? 1.f : -1.f;
+    const float densb = sg * (dirac0 + dirac1 * cr.y + dirac2 * cr.z + dirac3 * cr.w);
+    const float colb0 = sg * cr.x * dirac1, colb1 = sg * cr.x * dirac2, colb2 = sg * cr.x * dirac3;
+    const float4 q = reinterpret_cast<const float4*>(A.quats)[p];
+    const f3 mean = ld3(A.means + 3 * p);
+    const f3 scl = ld3(A.scales + 3 * p);
+    const f3 ro = org - mean;
+    SafeRoots S;
+    f3 rob = mk3(0.f, 0.f, 0.f), rdb = mk3(0.f, 0.f, 0.f), sbar = mk3(0.f, 0.f, 0.f);
+    float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (safe_roots(ro, dir, scl, q, S)) safe_roots_bwd(S, ro, dir, scl, q, face, tb, rob, rdb, sbar, qb);
+    orgb = orgb + rob;
+    dirb = dirb + rdb;
+    float* ac = A.acc + 16 * p;
+    atomic_addf(ac + 0, -rob.x);
+    atomic_addf(ac + 1, -rob.y);
+    atomic_addf(ac + 2, -rob.z);
+    atomic_addf(ac + 3, sbar.x);
+    atomic_addf(ac + 4, sbar.y);
+    atomic_addf(ac + 5, sbar.z);
+    atomic_addf(ac + 6, qb.x);
+    atomic_addf(ac + 7, qb.y);
+    atomic_addf(ac + 8, qb.z);
+    atomic_addf(ac + 9, qb.w);
+    atomic_addf(ac + 10, densb);
+    atomic_addf(ac + 11, colb0);
+    atomic_addf(ac + 12, colb1);
+    atomic_addf(ac + 13, colb2);
+    if (inv) {
+      // means2D statistic (backwards_kernel.slang:213-220) with this ray's matrix
+      const float dist = mean.x * W[2] + mean.y * W[6] + mean.z * W[10] + W[14];
+      atomic_addf(ac + 14, dist * (inv[0] * -rob.x + inv[1] * -rob.y + inv[2] * -rob.z));
+      atomic_addf(ac + 15, dist * (inv[4] * -rob.x + inv[5] * -rob.y + inv[6] * -rob.z));
+    }
+    // step to the previous event
+    Da0 = Dbf0; Da1 = Dbf1; Da2 = Dbf2; Da3 = Dbf3;
+    La = Lbf;
+    csxa = csxbf;
+    csya = csybf;
+    Lb = Lb_n;
+    csxb = csxb_n;
+    csyb = csyb_n;
+    Db0 = Db0_n; Db1 = Db1_n; Db2 = Db2_n; Db3 = Db3_n;
+    tcarry = tprevb;
+    id = id_prev;
+    tk = tprev;
+  }
+  *dinit = make_float4(Db0, Db1, Db2, Db3);
+  float* o = A.drayo + 3 * r;
+  float* dd = A.drayd + 3 * r;
+  o[0] = clipf(orgb.x, -1e3f, 1e3f);
+  o[1] = clipf(orgb.y, -1e3f, 1e3f);
+  o[2] = clipf(orgb.z, -1e3f, 1e3f);
+  dd[0] = clipf(dirb.x, -1e3f, 1e3f);
+  dd[1] = clipf(dirb.y, -1e3f, 1e3f);
+  dd[2] = clipf(dirb.z, -1e3f, 1e3f);
+}
+
+// backwards_initial_drgb_kernel (backwards_kernel.slang:374-428): ray 0's origin + tmin * raw direction.
+__global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ means, const float* __restrict__ scales,
+                                                      const float* __restrict__ quats, const float4* __restrict__ colr,
+                                                      const int32_t* __restrict__ cand,
+                                                      const uint32_t* __restrict__ header,
+                                                      const float* __restrict__ rayo, const float* __restrict__ rayd,
+                                                      int64_t nr, float tmin, const float* __restrict__ dinit,
+                                                      float* __restrict__ acc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int32_t ncand = (int32_t)header[8];
+  if (ncand == 0) return;
+  const float4 gv = reinterpret_cast<const float4*>(dinit)[r];
+  if (gv.x == 0.f && gv.y == 0.f && gv.z == 0.f && gv.w == 0.f) return;
+  const f3 p0 = ld3(rayo) + tmin * ld3(rayd + 3 * r);
+  for (int32_t j = 0; j < ncand; ++j) {
+    const int64_t p = cand[j];
+    float4 q = reinterpret_cast<const float4*>(quats)[p];
+    const float qn = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    q = make_float4(safe_div(q.x, qn), safe_div(q.y, qn), safe_div(q.z, qn), safe_div(q.w, qn));
+    const f3 v = qrot(p0 - ld3(means + 3 * p), q);
+    const f3 s = ld3(scales + 3 * p);
+    const f3 tr = mk3(safe_div(v.x, fmaxf(s.x, 1e-8f)), safe_div(v.y, fmaxf(s.y, 1e-8f)),
+                      safe_div(v.z, fmaxf(s.z, 1e-8f)));
+    if (sqrtf(dot3(tr, tr)) <= 1.f) {
+      const float4 cr = colr[p];
+      float* ac = acc + 16 * p;
+      atomic_addf(ac + 10, gv.x + gv.y * cr.y + gv.z * cr.z + gv.w * cr.w);
+      atomic_addf(ac + 11, cr.x * gv.y);
+      atomic_addf(ac + 12, cr.x * gv.z);
+      atomic_addf(ac + 13, cr.x * gv.w);
+    }
+  }
+}
+
+struct EpiArgs {
+  const float* acc;
+  const float* means;
+  const float* feat;
+  int64_t n;
+  int d;
+  int color_mode, sh_degree, activation;
+  float ox, oy, oz;
+  const float* wcts;
+  const float* inv;
+  int64_t nw;
+  float* dmeans;
+  float* dscales;
+  float* dquats;
+  float* ddens;
+  float* dfeat;
+  float* dmeans2D;
+};
+
+// Per primitive: clamps (fast_ellipsoid_splinetracer.py:201-211), colour -> SH features, means2D.
+__global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E.n) return;
+  const float4* a4 = reinterpret_cast<const float4*>(E.acc + 16 * p);
+  const float4 a0 = a4[0], a1 = a4[1], a2 = a4[2], a3 = a4[3];
+  // a0 = (mx,my,mz,sx) a1 = (sy,sz,qw,qx) a2 = (qy,qz,dens,c0) a3 = (c1,c2,m2x,m2y)
+  E.dmeans[3 * p + 0] = clipf(a0.x, -1e3f, 1e3f);
+  E.dmeans[3 * p + 1] = clipf(a0.y, -1e3f, 1e3f);
+  E.dmeans[3 * p + 2] = clipf(a0.z, -1e3f, 1e3f);
+  E.dscales[3 * p + 0] = clipf(a0.w, -1e3f, 1e3f);
+  E.dscales[3 * p + 1] = clipf(a1.x, -1e3f, 1e3f);
+  E.dscales[3 * p + 2] = clipf(a1.y, -1e3f, 1e3f);
+  E.dquats[4 * p + 0] = clipf(a1.z, -1e3f, 1e3f);
+  E.dquats[4 * p + 1] = clipf(a1.w, -1e3f, 1e3f);
+  E.dquats[4 * p + 2] = clipf(a2.x, -1e3f, 1e3f);
+  E.dquats[4 * p + 3] = clipf(a2.y, -1e3f, 1e3f);
+  E.ddens[p] = clipf(a2.z, -50.f, 50.f);
+  // dL/d(deg-0 feature) = SH_C0 * dL/dcolour, clamped like the reference's feature gradient
+  const float g0 = clipf(kShC0 * a2.w, -1e3f, 1e3f), g1 = clipf(kShC0 * a3.x, -1e3f, 1e3f),
+              g2 = clipf(kShC0 * a3.y, -1e3f, 1e3f);
+  float* df = E.dfeat + (int64_t)3 * E.d * p;
+  if (E.color_mode == EVER_COLOR_SH) {
+    // chain through RGB2SH (1/SH_C0), the optional softplus and the SH basis (sh_kernel.slang:45-98)
+    const f3 mean = ld3(E.means + 3 * p);
+    const f3 dir = sh_dir(mean, mk3(E.ox, E.oy, E.oz));
+    float bas[16];
+    sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
+    const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
+    float gr[3] = {g0 / kShC0, g1 / kShC0, g2 / kShC0};
+    if (E.activation == 1) {
+      const float* f = E.feat + (int64_t)3 * E.d * p;
+      for (int ch = 0; ch < 3; ++ch) {
+        float x = 0.5f;
+        for (int k = 0; k < nc; ++k) x += bas[k] * f[3 * k + ch];
+        gr[ch] *= softplus10_grad(x);
+      }
+    }
+    const bool skip = fabsf(gr[0]) < 1e-8f && fabsf(gr[1]) < 1e-8f && fabsf(gr[2]) < 1e-8f;
+    for (int k = 0; k < E.d; ++k)
+      for (int ch = 0; ch < 3; ++ch) df[3 * k + ch] = (!skip && k < nc) ? bas[k] * gr[ch] : 0.f;
+  } else {
+    df[0] = g0;
+    df[1] = g1;
+    df[2] = g2;
+    for (int k = 3; k < 3 * E.d; ++k) df[k] = 0.f;
+  }
+  if (E.dmeans2D) {
+    float m2x, m2y;
+    if (E.nw == 1) {
+      // the statistic is linear in the per-event mean gradient: apply it to the unclamped total
+      const f3 mean = ld3(E.means + 3 * p);
+      const float* W = E.wcts;
+      const float* iv = E.inv;
+      const float dist = mean.x * W[2] + mean.y * W[6] + mean.z * W[10] + W[14];
+      m2x = dist * (iv[0] * a0.x + iv[1] * a0.y + iv[2] * a0.z);
+      m2y = dist * (iv[4] * a0.x + iv[5] * a0.y + iv[6] * a0.z);
+    } else {
+      m2x = a3.z;
+      m2y = a3.w;
+    }
+    E.dmeans2D[2 * p + 0] = m2x;
+    E.dmeans2D[2 * p + 1] = m2y;
+  }
+}
+
+hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
+                           hipStream_t st) {
+  const int64_t n = P.num_prims, nr = R.num_rays;
+  SceneLayout L = scene_layout(n);
+  BackwardLayout BL = backward_layout(n, nr, B.wcts ? B.num_wcts : 0);
+  float* acc = at<float>(B.workspace, BL.acc);
+  float* inv = at<float>(B.workspace, BL.inv);
+  float* dinit = B.dinitial_drgb ? B.dinitial_drgb : at<float>(B.workspace, BL.dinit);
+  const int64_t nw = B.wcts ? B.num_wcts : 0;
+  if (n > 0) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(16 * n, 256)), dim3(256), 0, st, acc, 16 * n);
+  if (nw > 0) hipLaunchKernelGGL(k_invert4, dim3(grid_for(nw, 64)), dim3(64), 0, st, B.wcts, nw, inv);
+  if (nr > 0) {
+    BwdArgs A;
+    A.means = P.means;
+    A.scales = P.scales;
+    A.quats = P.quats;
+    A.colr = at<float4>(ws, L.colr);
+    A.rayo = R.origins;
+    A.rayd = R.directions;
+    A.nr = nr;
+    A.tmin = R.tmin;
+    A.max_iters = R.max_iters;
+    A.grad_out = B.grad_out;
+    A.iters = B.iters;
+    A.tri = B.tri_collection;
+    A.states = B.states;
+    A.st64 = B.ray_state64;
+    A.wcts = B.wcts;
+    A.nw = nw;
+    A.inv = inv;
+    A.acc = acc;
+    A.drayo = B.drayo;
+    A.drayd = B.drayd;
+    A.dinit = dinit;
+    hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
+    if (n > 0)
+      hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
+                         at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,
+                         R.directions, nr, R.tmin, dinit, acc);
+  }
+  if (n > 0) {
+    EpiArgs E;
+    E.acc = acc;
+    E.means = P.means;
+    E.feat = P.features;
+    E.n = n;
+    E.d = P.feature_size;
+    E.color_mode = P.color_mode;
+    E.sh_degree = P.sh_degree;
+    E.activation = P.activation;
+    E.ox = P.sh_origin[0];
+    E.oy = P.sh_origin[1];
+    E.oz = P.sh_origin[2];
+    E.wcts = B.wcts;
+    E.inv = inv;
+    E.nw = nw;
+    E.dmeans = B.dmeans;
+    E.dscales = B.dscales;
+    E.dquats = B.dquats;
+    E.ddens = B.ddensities;
+    E.dfeat = B.dfeatures;
+    E.dmeans2D = nw > 0 ? B.dmeans2D : nullptr;
+    hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ever

@@ -1,0 +1,372 @@
+// LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (AABB, colour, centroid
+// bounds), 30-bit Morton codes, an LDS radix sort written for wave64, Karras' parallel hierarchy
+// construction and an atomic-flag bottom-up refit that also writes the per-primitive traversal records.
+//
+// Replaces the reference's OptiX acceleration structure (splinetracers/fast_ellipsoid_splinetracer/GAS.cpp:56-153)
+// and its AABB kernel (create_aabbs.cu:69-115), plus the per-primitive SH prepass (splinetracers/slang/sh_kernel.slang:19-42)
+// when colour mode EVER_COLOR_SH is selected.
+#include "ever_common.hpp"
+#include "ever_internal.hpp"
+
+namespace ever {
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_build_init(uint32_t* header) {
+  int t = threadIdx.x;
+  if (t < 3) header[t] = 0xffffffffu;   // min (ordered)
+  else if (t < 6) header[t] = 0u;       // max (ordered)
+  else if (t < 16) header[t] = 0u;
+}
+
+// Per primitive (original order): AABB of R(q/|q|) diag(s) applied to the unit ball (create_aabbs.cu:77-114),
+// padded by a relative 1e-6 so the box stays conservative under rounding; colour record; centroid bounds.
+__global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float* __restrict__ aabb,
+                                                      float4* __restrict__ colr, uint32_t* __restrict__ header) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = i < P.num_prims;
+  float cx = 0.f, cy = 0.f, cz = 0.f;
+  if (valid) {
+    float4 q = reinterpret_cast<const float4*>(P.quats)[i];
+    float qn = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    float r = q.x * qn, x = q.y * qn, y = q.z * qn, z = q.w * qn;
+    float R00 = 1.f - 2.f * (y * y + z * z), R01 = 2.f * (x * y - r * z), R02 = 2.f * (x * z + r * y);
+    float R10 = 2.f * (x * y + r * z), R11 = 1.f - 2.f * (x * x + z * z), R12 = 2.f * (y * z - r * x);
+    float R20 = 2.f * (x * z - r * y), R21 = 2.f * (y * z + r * x), R22 = 1.f - 2.f * (x * x + y * y);
+    f3 s = ld3(P.scales + 3 * i);
+    f3 m = ld3(P.means + 3 * i);
+    float ex = sqrtf(R00 * R00 * s.x * s.x + R01 * R01 * s.y * s.y + R02 * R02 * s.z * s.z);
+    float ey = sqrtf(R10 * R10 * s.x * s.x + R11 * R11 * s.y * s.y + R12 * R12 * s.z * s.z);
+    float ez = sqrtf(R20 * R20 * s.x * s.x + R21 * R21 * s.y * s.y + R22 * R22 * s.z * s.z);
+    ex = ex * (1.f + 1e-6f) + 1e-7f * fabsf(m.x);
+    ey = ey * (1.f + 1e-6f) + 1e-7f * fabsf(m.y);
+    ez = ez * (1.f + 1e-6f) + 1e-7f * fabsf(m.z);
+    float* b = aabb + 6 * i;
+    b[0] = m.x - ex; b[1] = m.y - ey; b[2] = m.z - ez;
+    b[3] = m.x + ex; b[4] = m.y + ey; b[5] = m.z + ez;
+    cx = m.x; cy = m.y; cz = m.z;
+    float dens = P.densities[i];
+    float3 c;
+    const float* f = P.features + (int64_t)3 * P.feature_size * i;
+    if (P.color_mode == EVER_COLOR_SH) {
+      f3 dir = sh_dir(m, mk3(P.sh_origin[0], P.sh_origin[1], P.sh_origin[2]));
+      float bas[16];
+      sh_basis(P.sh_degree, dir.x, dir.y, dir.z, bas);
+      int nc = (P.sh_degree + 1) * (P.sh_degree + 1);
+      float acc[3] = {0.5f, 0.5f, 0.5f};
+      for (int k = 0; k < nc; ++k)
+        for (int ch = 0; ch < 3; ++ch) acc[ch] += bas[k] * f[3 * k + ch];
+      if (P.activation == 1)
+        for (int ch = 0; ch < 3; ++ch) acc[ch] = softplus10(acc[ch]);
+      // colour of the deg-0 feature RGB2SH(rgb) seen through SH_C0 * f + 0.5 (ever.py:178-180, shaders.slang:65-82)
+      c = make_float3(acc[0], acc[1], acc[2]);
+    } else {
+      c = make_float3(kShC0 * f[0] + 0.5f, kShC0 * f[1] + 0.5f, kShC0 * f[2] + 0.5f);
+    }
+    colr[i] = make_float4(dens, c.x, c.y, c.z);
+  }
+  // wave-level reduction of centroid bounds, one atomic per wave and component
+  float mnx = valid ? cx : INFINITY, mny = valid ? cy : INFINITY, mnz = valid ? cz : INFINITY;
+  float mxx = valid ? cx : -INFINITY, mxy = valid ? cy : -INFINITY, mxz = valid ? cz : -INFINITY;
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fminf(mnx, __shfl_xor(mnx, o));
+    mny = fminf(mny, __shfl_xor(mny, o));
+    mnz = fminf(mnz, __shfl_xor(mnz, o));
+    mxx = fmaxf(mxx, __shfl_xor(mxx, o));
+    mxy = fmaxf(mxy, __shfl_xor(mxy, o));
+    mxz = fmaxf(mxz, __shfl_xor(mxz, o));
+  }
+  if ((threadIdx.x & 63) == 0 && mnx <= mxx) {
+    atomicMin(header + 0, f2o(mnx));
+    atomicMin(header + 1, f2o(mny));
+    atomicMin(header + 2, f2o(mnz));
+    atomicMax(header + 3, f2o(mxx));
+    atomicMax(header + 4, f2o(mxy));
+    atomicMax(header + 5, f2o(mxz));
+  }
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ aabb, int64_t n,
+                                                const uint32_t* __restrict__ header, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float lo[3] = {o2f(header[0]), o2f(header[1]), o2f(header[2])};
+  float hi[3] = {o2f(header[3]), o2f(header[4]), o2f(header[5])};
+  const float* b = aabb + 6 * i;
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    float ext = hi[k] - lo[k];
+    float c = 0.5f * (b[k] + b[3 + k]);
+    float u = ext > 0.f ? (c - lo[k]) / ext : 0.5f;
+    q[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+  }
+  keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+  vals[i] = (uint32_t)i;
+}
+
+// ---- LSD radix sort, 8-bit digits, tiles of kSortTile keys --------------------------------------------------
+// Histogram: hist is digit-major, hist[digit * tiles + tile].
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                             uint32_t* __restrict__ hist, int64_t tiles) {
+  __shared__ uint32_t h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    int64_t idx = base + it * kSortThreads + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of hist (length m) by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ a, int64_t m) {
+  __shared__ uint32_t part[1024];
+  int t = threadIdx.x;
+  int64_t chunk = (m + 1023) / 1024;
+  int64_t b = t * chunk, e = b + chunk < m ? b + chunk : m;
+  uint32_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += a[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+  for (int64_t i = b; i < e; ++i) {
+    uint32_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+}
+
+// Stable scatter: 16 rounds of 256 consecutive keys; ranks within a wave from 8 ballots (multi-split),
+// across the 4 waves of the block from an LDS prefix.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                uint32_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout, int64_t n, int shift,
+                                                                const uint32_t* __restrict__ hist, int64_t tiles) {
+  __shared__ uint32_t offs[kRadix];
+  __shared__ uint32_t cnt[4][kRadix];
+  __shared__ uint32_t tot[kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  offs[tid] = hist[(int64_t)tid * tiles + blockIdx.x];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    int64_t idx = base + it * kSortThreads + tid;
+    bool valid = idx < n;
+    uint32_t key = valid ? kin[idx] : 0u;
+    uint32_t val = valid ? vin[idx] : 0u;
+    uint32_t digit = (key >> shift) & 0xffu;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (digit >> b) & 1u;
+      uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t count = (uint32_t)__popcll(peers);
+    cnt[0][tid] = 0;
+    cnt[1][tid] = 0;
+    cnt[2][tid] = 0;
+    cnt[3][tid] = 0;
+    __syncthreads();
+    if (valid && rank == 0) cnt[wave][digit] = count;
+    __syncthreads();
+    {
+      uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid], c3 = cnt[3][tid];
+      cnt[0][tid] = 0;
+      cnt[1][tid] = c0;
+      cnt[2][tid] = c0 + c1;
+      cnt[3][tid] = c0 + c1 + c2;
+      tot[tid] = c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = offs[digit] + cnt[wave][digit] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    offs[tid] += tot[tid];
+    __syncthreads();
+  }
+}
+
+// ---- Karras (2012) hierarchy over the sorted codes ---------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint32_t* keys, int64_t n, int64_t a, int64_t b) {
+  if (b < 0 || b >= n) return -1;
+  uint32_t ka = keys[a], kb = keys[b];
+  if (ka != kb) return __clz(ka ^ kb);
+  return 32 + __clz((uint32_t)a ^ (uint32_t)b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int64_t n,
+                                                float4* __restrict__ nodes, int32_t* __restrict__ parent,
+                                                int32_t* __restrict__ leaf_parent) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = kdelta(keys, n, i, i - d);
+  int64_t lmax = 2;
+  while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int64_t j = i + l * d;
+  int dnode = kdelta(keys, n, i, j);
+  int64_t s = 0;
+  int64_t t = l;
+  do {
+    t = (t + 1) / 2;
+    if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int64_t gamma = i + s * d + (d < 0 ? d : 0);
+  int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  int32_t left, right;
+  if (lo == gamma) {
+    left = ~(int32_t)gamma;
+    leaf_parent[gamma] = (int32_t)i;
+  } else {
+    left = (int32_t)gamma;
+    parent[gamma] = (int32_t)i;
+  }
+  if (hi == gamma + 1) {
+    right = ~(int32_t)(gamma + 1);
+    leaf_parent[gamma + 1] = (int32_t)i;
+  } else {
+    right = (int32_t)(gamma + 1);
+    parent[gamma + 1] = (int32_t)i;
+  }
+  nodes[4 * i + 3] = make_float4(__int_as_float(left), __int_as_float(right), 0.f, 0.f);
+  if (i == 0) parent[0] = -1;
+}
+
+// Per sorted leaf: traversal record, then bottom-up refit. Each internal node holds its two children's boxes;
+// the second child to arrive (atomic counter) merges them and continues to the parent.
+//   record (64 B): rows (M_k, b_k), k = 0..2, with M = diag(1/s) L_q, b = -M mean, so that the ellipsoid-frame
+//   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
+//   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
+//   row 3 = (density, R, G, B).
+__global__ __launch_bounds__(256) void k_leaves_refit(ever_prims P, const float* __restrict__ aabb,
+                                                      const float4* __restrict__ colr,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ leaf_parent,
+                                                      float4* __restrict__ nodes, uint32_t* __restrict__ flags,
+                                                      float4* __restrict__ rec) {
+  int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t n = P.num_prims;
+  if (sidx >= n) return;
+  uint32_t i = perm[sidx];
+  {
+    float4 q = reinterpret_cast<const float4*>(P.quats)[i];
+    f3 s = ld3(P.scales + 3 * (int64_t)i);
+    f3 m = ld3(P.means + 3 * (int64_t)i);
+    f3 c0 = qrot(mk3(1.f, 0.f, 0.f), q), c1 = qrot(mk3(0.f, 1.f, 0.f), q), c2 = qrot(mk3(0.f, 0.f, 1.f), q);
+    float M[3][3] = {{c0.x / s.x, c1.x / s.x, c2.x / s.x},
+                     {c0.y / s.y, c1.y / s.y, c2.y / s.y},
+                     {c0.z / s.z, c1.z / s.z, c2.z / s.z}};
+    for (int k = 0; k < 3; ++k) {
+      float b = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
+      rec[4 * sidx + k] = make_float4(M[k][0], M[k][1], M[k][2], b);
+    }
+    rec[4 * sidx + 3] = colr[i];
+  }
+  if (n < 2) return;
+  const float* b = aabb + 6 * (int64_t)i;
+  float lo0 = b[0], lo1 = b[1], lo2 = b[2], hi0 = b[3], hi1 = b[4], hi2 = b[5];
+  int32_t child = ~(int32_t)sidx;
+  int32_t node = leaf_parent[sidx];
+  while (node >= 0) {
+    float* nf = reinterpret_cast<float*>(nodes + 4 * (int64_t)node);
+    int32_t c0 = __float_as_int(nf[12]);
+    int slot = (c0 == child) ? 0 : 6;
+    nf[slot + 0] = lo0; nf[slot + 1] = lo1; nf[slot + 2] = lo2;
+    nf[slot + 3] = hi0; nf[slot + 4] = hi1; nf[slot + 5] = hi2;
+    uint32_t old = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    int other = 6 - slot;
+    float olo0 = __hip_atomic_load(nf + other + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float olo1 = __hip_atomic_load(nf + other + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float olo2 = __hip_atomic_load(nf + other + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float ohi0 = __hip_atomic_load(nf + other + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float ohi1 = __hip_atomic_load(nf + other + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float ohi2 = __hip_atomic_load(nf + other + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    lo0 = fminf(lo0, olo0); lo1 = fminf(lo1, olo1); lo2 = fminf(lo2, olo2);
+    hi0 = fmaxf(hi0, ohi0); hi1 = fmaxf(hi1, ohi1); hi2 = fmaxf(hi2, ohi2);
+    child = node;
+    node = parent[node];
+  }
+}
+
+hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
+  const int64_t n = P.num_prims;
+  SceneLayout L = scene_layout(n);
+  uint32_t* header = at<uint32_t>(ws, L.header);
+  float* aabb = at<float>(ws, L.aabb);
+  uint32_t* k0 = at<uint32_t>(ws, L.keys0);
+  uint32_t* v0 = at<uint32_t>(ws, L.vals0);
+  uint32_t* k1 = at<uint32_t>(ws, L.keys1);
+  uint32_t* v1 = at<uint32_t>(ws, L.vals1);
+  uint32_t* hist = at<uint32_t>(ws, L.hist);
+  float4* nodes = at<float4>(ws, L.nodes);
+  int32_t* parent = at<int32_t>(ws, L.parent);
+  int32_t* leaf_parent = at<int32_t>(ws, L.leaf_parent);
+  uint32_t* flags = at<uint32_t>(ws, L.flags);
+  float4* rec = at<float4>(ws, L.rec);
+  float4* colr = at<float4>(ws, L.colr);
+
+  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, header);
+  if (n <= 0) return hipGetLastError();
+  hipLaunchKernelGGL(k_prim_prepass, dim3(grid_for(n, 256)), dim3(256), 0, st, P, aabb, colr, header);
+  hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, st, aabb, n, header, k0, v0);
+  const int64_t tiles = L.tiles;
+  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+  for (int shift = 0; shift < 30; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, (int64_t)kRadix * tiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
+                       shift, hist, tiles);
+    uint32_t* t = kin; kin = kout; kout = t;
+    t = vin; vin = vout; vout = t;
+  }
+  // four passes: sorted keys/permutation are back in keys0/vals0
+  if (n > 1) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, flags, n - 1, 0u);
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, kin, n, nodes, parent, leaf_parent);
+  }
+  hipLaunchKernelGGL(k_leaves_refit, dim3(grid_for(n, 256)), dim3(256), 0, st, P, aabb, colr, vin, parent,
+                     leaf_parent, nodes, flags, rec);
+  return hipGetLastError();
+}
+
+}  // namespace ever

@@ -1,0 +1,153 @@
+// C ABI entry points (include/ever_amd.h): argument validation, workspace checks, status codes.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ever_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return EVER_OK;
+  return fail(EVER_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int check_prims(const ever_prims* P) {
+  if (!P) return fail(EVER_ERR_INVALID, "prims is null");
+  if (P->num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (P->num_prims >= (int64_t(1) << 30))
+    return fail(EVER_ERR_INVALID, "num_prims must be < 2^30 (event ids are 2*prim+face in int32)");
+  if (P->num_prims > 0 && (!P->means || !P->scales || !P->quats || !P->densities || !P->features))
+    return fail(EVER_ERR_INVALID, "a primitive array is null");
+  if (P->feature_size < 1) return fail(EVER_ERR_INVALID, "feature_size must be >= 1");
+  if (P->color_mode != EVER_COLOR_DC && P->color_mode != EVER_COLOR_SH)
+    return fail(EVER_ERR_INVALID, "unknown color_mode");
+  if (P->color_mode == EVER_COLOR_SH) {
+    if (P->sh_degree < 0 || P->sh_degree > 3) return fail(EVER_ERR_INVALID, "sh_degree must be in [0, 3]");
+    if ((P->sh_degree + 1) * (P->sh_degree + 1) > P->feature_size)
+      return fail(EVER_ERR_INVALID, "feature_size is smaller than (sh_degree+1)^2");
+    if (P->activation != 0 && P->activation != 1) return fail(EVER_ERR_INVALID, "unknown activation");
+  }
+  return EVER_OK;
+}
+
+int check_rays(const ever_rays* R) {
+  if (!R) return fail(EVER_ERR_INVALID, "rays is null");
+  if (R->num_rays < 0) return fail(EVER_ERR_INVALID, "num_rays must be >= 0");
+  if (R->num_rays > 0 && (!R->origins || !R->directions)) return fail(EVER_ERR_INVALID, "ray arrays are null");
+  if (R->max_iters < 0) return fail(EVER_ERR_INVALID, "max_iters must be >= 0");
+  if (R->kbuf != 8 && R->kbuf != 16 && R->kbuf != 32) return fail(EVER_ERR_INVALID, "kbuf must be 8, 16 or 32");
+  return EVER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ever_abi_version(void) { return EVER_ABI_VERSION; }
+
+int ever_struct_sizes(int64_t* out4) {
+  if (!out4) return fail(EVER_ERR_INVALID, "out4 is null");
+  out4[0] = sizeof(ever_prims);
+  out4[1] = sizeof(ever_rays);
+  out4[2] = sizeof(ever_forward_out);
+  out4[3] = sizeof(ever_backward_io);
+  return EVER_OK;
+}
+
+const char* ever_last_error(void) { return g_last_error.c_str(); }
+
+size_t ever_scene_workspace_bytes(int64_t num_prims) {
+  return ever::scene_layout(num_prims > 0 ? num_prims : 0).total;
+}
+
+size_t ever_forward_workspace_bytes(int64_t num_rays) {
+  return ever::align_up(sizeof(int32_t) * ever::kSpillPerRay * (size_t)(num_rays > 0 ? num_rays : 1), 256);
+}
+
+size_t ever_backward_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts) {
+  return ever::backward_layout(num_prims, num_rays, num_wcts).total;
+}
+
+int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream) {
+  if (int rc = check_prims(prims)) return rc;
+  if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
+  if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
+    return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
+  return hip_status(ever::launch_build(*prims, workspace, (hipStream_t)stream), "ever_build_scene");
+}
+
+int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
+                       const ever_forward_out* out, void* stream) {
+  if (int rc = check_prims(prims)) return rc;
+  if (int rc = check_rays(rays)) return rc;
+  if (!out) return fail(EVER_ERR_INVALID, "out is null");
+  if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
+  if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
+    return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
+  if (rays->num_rays > 0) {
+    if (!out->color || !out->states || !out->iters || !out->initial_drgb || !out->ray_state64)
+      return fail(EVER_ERR_INVALID, "a required forward output is null");
+    if (!out->workspace || out->workspace_bytes < ever_forward_workspace_bytes(rays->num_rays))
+      return fail(EVER_ERR_WORKSPACE, "forward workspace too small");
+  }
+  return hip_status(ever::launch_forward(*prims, workspace, *rays, *out, static_cast<int32_t*>(out->workspace),
+                                         (hipStream_t)stream),
+                    "ever_trace_forward");
+}
+
+int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
+                        const ever_backward_io* io, void* stream) {
+  if (int rc = check_prims(prims)) return rc;
+  if (int rc = check_rays(rays)) return rc;
+  if (!io) return fail(EVER_ERR_INVALID, "io is null");
+  if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
+  if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
+    return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
+  const int64_t nw = io->wcts ? io->num_wcts : 0;
+  if (io->wcts && io->num_wcts < 1) return fail(EVER_ERR_INVALID, "num_wcts must be >= 1 when wcts is given");
+  if (io->wcts && !io->dmeans2D) return fail(EVER_ERR_INVALID, "dmeans2D is null while wcts is given");
+  if (!io->workspace ||
+      io->workspace_bytes < ever_backward_workspace_bytes(prims->num_prims, rays->num_rays, nw))
+    return fail(EVER_ERR_WORKSPACE, "backward workspace too small");
+  if (prims->num_prims > 0 && (!io->dmeans || !io->dscales || !io->dquats || !io->ddensities || !io->dfeatures))
+    return fail(EVER_ERR_INVALID, "a primitive gradient output is null");
+  if (rays->num_rays > 0 && (!io->grad_out || !io->iters || !io->states || !io->ray_state64 || !io->drayo ||
+                             !io->drayd || !io->tri_collection))
+    return fail(EVER_ERR_INVALID, "a ray input/output of the backward is null (stored events are required)");
+  return hip_status(ever::launch_backward(*prims, workspace, *rays, *io, (hipStream_t)stream), "ever_trace_backward");
+}
+
+int ever_eval_sh(const float* means, const float* features, const float* origin, int64_t num_prims,
+                 int32_t feature_size, int32_t sh_degree, float* out_rgb, void* stream) {
+  if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (sh_degree < 0 || sh_degree > 3) return fail(EVER_ERR_INVALID, "sh_degree must be in [0, 3]");
+  if ((sh_degree + 1) * (sh_degree + 1) > feature_size)
+    return fail(EVER_ERR_INVALID, "feature_size is smaller than (sh_degree+1)^2");
+  if (num_prims > 0 && (!means || !features || !origin || !out_rgb)) return fail(EVER_ERR_INVALID, "null pointer");
+  return hip_status(ever::launch_eval_sh(means, features, origin, num_prims, feature_size, sh_degree, out_rgb,
+                                         (hipStream_t)stream),
+                    "ever_eval_sh");
+}
+
+int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,
+                          int64_t num_prims, int32_t feature_size, int32_t sh_degree, float* dfeatures,
+                          void* stream) {
+  if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (sh_degree < 0 || sh_degree > 3) return fail(EVER_ERR_INVALID, "sh_degree must be in [0, 3]");
+  if ((sh_degree + 1) * (sh_degree + 1) > feature_size)
+    return fail(EVER_ERR_INVALID, "feature_size is smaller than (sh_degree+1)^2");
+  if (num_prims > 0 && (!means || !origin || !drgb || !dfeatures)) return fail(EVER_ERR_INVALID, "null pointer");
+  return hip_status(ever::launch_eval_sh_bwd(means, features, origin, drgb, num_prims, feature_size, sh_degree,
+                                             dfeatures, (hipStream_t)stream),
+                    "ever_eval_sh_backward");
+}
+
+}  // extern "C"

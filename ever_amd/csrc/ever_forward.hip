@@ -1,0 +1,339 @@
+// Forward tracer: one lane per ray, LBVH traversal with a short LDS stack (global spill beyond 32 entries),
+// a per-ray register k-buffer of the K nearest enter/exit events beyond the last processed depth, and the
+// exact piecewise-constant integrator applied to those events in order; re-traversal until the ray's
+// log-transmittance reaches 5.54, max_iters events were processed or no event is left.
+//
+// Semantics follow the reference's OptiX programs (splinetracers/fast_ellipsoid_splinetracer/slang/shaders.slang):
+//   intersection + report rules :84-107, any-hit k-buffer insertion (strict <) :117-164, raygen loop :171-235,
+//   integrator spline-machine.slang:103-153, start-inside density initialize_density.cu:69-150.
+// Differences, all toward the FP64 oracle: the density/colour accumulator and log-transmittance are kept in
+// FP64 (exactly invertible by the adjoint), 1-exp(-x) is evaluated as -expm1(-x), and touch counts use atomics.
+#include "ever_common.hpp"
+#include "ever_internal.hpp"
+
+namespace ever {
+
+constexpr int kFwdBlock = 64;
+constexpr int kLdsDepth = 32;     // LDS stack entries per lane
+constexpr int kSpillDepth = 32;   // global spill entries per ray (depth bound of a 30-bit LBVH: 30 + 32)
+
+struct FwdArgs {
+  const float4* rec;
+  const float4* nodes;
+  const uint32_t* perm;
+  const float4* colr;
+  const int32_t* cand;
+  const uint32_t* header;
+  const float* means;
+  const float* scales;
+  const float* quats;
+  const float* dens;
+  const float* feat;
+  int color_mode;
+  int64_t n;
+  const float* rayo;
+  const float* rayd;
+  int64_t nr;
+  float tmin, tmax;
+  int max_iters;
+  float* color;
+  float* states;
+  int32_t* iters;
+  int32_t* tri;
+  int32_t* touch;
+  float* init_drgb;
+  double* st64;
+  int32_t* spill;
+};
+
+// kern_prefilter (initialize_density.cu:69-105): primitives whose AABB lies within tmin of ray 0's origin.
+__global__ __launch_bounds__(256) void k_prefilter(const float* __restrict__ aabb, int64_t n,
+                                                   const float* __restrict__ rayo, float tmin,
+                                                   int32_t* __restrict__ cand, uint32_t* __restrict__ header) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* b = aabb + 6 * i;
+  float o[3] = {rayo[0], rayo[1], rayo[2]};
+  float dmin = 0.f;
+  for (int k = 0; k < 3; ++k) {
+    if (o[k] < b[k]) dmin += (o[k] - b[k]) * (o[k] - b[k]);
+    else if (o[k] > b[3 + k]) dmin += (o[k] - b[3 + k]) * (o[k] - b[3 + k]);
+  }
+  if (dmin <= tmin * tmin) {
+    uint32_t pos = atomicAdd(header + 8, 1u);
+    cand[pos] = (int32_t)i;
+  }
+}
+
+__global__ void k_reset_count(uint32_t* header) { header[8] = 0u; }
+
+__global__ void k_zero_i32(int32_t* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <int K>
+__device__ __forceinline__ void kbuf_insert(float (&bt)[K], uint32_t (&bi)[K], float t, uint32_t id) {
+  if (!(t < bt[K - 1])) return;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bool sw = t < bt[k];
+    float tt = bt[k];
+    uint32_t ii = bi[k];
+    bt[k] = sw ? t : tt;
+    bi[k] = sw ? id : ii;
+    t = sw ? tt : t;
+    id = sw ? ii : id;
+  }
+}
+
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
+                                     float t_lo, float t_hi, float& t_near) {
+  float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+  float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+  float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+  float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), t_lo));
+  float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t_hi));
+  t_near = tn;
+  return tn <= tf;
+}
+
+template <int K>
+__global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
+  __shared__ int32_t stk[kLdsDepth * kFwdBlock];
+  const int lane = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kFwdBlock + lane;
+  if (r >= A.nr) return;
+
+  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
+  const f3 d = l2n(rdraw);
+  const f3 o = ro + A.tmin * rdraw;  // shaders.slang:178-179: origin shifted along the raw direction
+  const f3 inv = mk3(fabsf(d.x) > 1e-20f ? 1.f / d.x : copysignf(1e20f, d.x),
+                     fabsf(d.y) > 1e-20f ? 1.f / d.y : copysignf(1e20f, d.y),
+                     fabsf(d.z) > 1e-20f ? 1.f / d.z : copysignf(1e20f, d.z));
+
+  // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
+  double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
+  {
+    const int32_t ncand = (int32_t)A.header[8];
+    const f3 p = ro + A.tmin * d;
+    for (int32_t j = 0; j < ncand; ++j) {
+      const int64_t i = A.cand[j];
+      float4 q = reinterpret_cast<const float4*>(A.quats)[i];
+      float qn = 1.f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+      q = make_float4(q.x * qn, q.y * qn, q.z * qn, q.w * qn);
+      f3 v = qrot(p - ld3(A.means + 3 * i), q);
+      f3 s = ld3(A.scales + 3 * i);
+      f3 tr = mk3(v.x / s.x, v.y / s.y, v.z / s.z);
+      if (dot3(tr, tr) <= 1.f) {
+        float den = A.dens[i];
+        float c0, c1, c2;
+        if (A.color_mode == EVER_COLOR_SH) {
+          float4 cr = A.colr[i];
+          c0 = cr.y; c1 = cr.z; c2 = cr.w;
+        } else {  // the reference reads the features at flat index prim*3 + c (initialize_density.cu:140-142)
+          c0 = A.feat[3 * i + 0] * kShC0 + 0.5f;
+          c1 = A.feat[3 * i + 1] * kShC0 + 0.5f;
+          c2 = A.feat[3 * i + 2] * kShC0 + 0.5f;
+        }
+        D0 += den;
+        D1 += den * c0;
+        D2 += den * c1;
+        D3 += den * c2;
+      }
+    }
+  }
+  reinterpret_cast<float4*>(A.init_drgb)[r] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
+
+  float t_state = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, csx = 0.f, csy = 0.f, dpx = 0.f, dpy = 0.f;
+  double logT = 0.0;
+  int iter = 0;
+  const int32_t root = A.n == 1 ? ~0 : 0;
+  int32_t* spill = A.spill + r * kSpillDepth;
+
+  while (logT < (double)kLogCutoff && iter < A.max_iters) {
+    const float cur_t = fabsf(t_state);
+    float bt[K];
+    uint32_t bi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      bt[k] = kEmptyT;
+      bi[k] = 0u;
+    }
+    // ---- traversal: gather the K nearest events beyond cur_t ----
+    int sp = 0;
+    int32_t node = root;
+    while (true) {
+      if (node >= 0) {
+        const float4* nd = A.nodes + 4 * (int64_t)node;
+        const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+        const float t_hi = fminf(A.tmax, bt[K - 1]);
+        float tn0, tn1;
+        const bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, cur_t, t_hi, tn0);
+        const bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, cur_t, t_hi, tn1);
+        const int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+        if (h0 && h1) {
+          const bool first0 = tn0 <= tn1;
+          const int32_t farc = first0 ? c1 : c0;
+          if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = farc;
+          else spill[sp - kLdsDepth] = farc;
+          ++sp;
+          node = first0 ? c0 : c1;
+          continue;
+        }
+        if (h0) { node = c0; continue; }
+        if (h1) { node = c1; continue; }
+      } else {
+        // leaf: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame
+        const uint32_t s = (uint32_t)(~node);
+        const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
+        const f3 ocn = mk3(RA.x * o.x + RA.y * o.y + RA.z * o.z + RA.w, RB.x * o.x + RB.y * o.y + RB.z * o.z + RB.w,
+                           RC.x * o.x + RC.y * o.y + RC.z * o.z + RC.w);
+        const f3 rdn = mk3(RA.x * d.x + RA.y * d.y + RA.z * d.z, RB.x * d.x + RB.y * d.y + RB.z * d.z,
+                           RC.x * d.x + RC.y * d.y + RC.z * d.z);
+        const float aa = dot3(rdn, rdn);
+        const float bp = -dot3(ocn, rdn);
+        const f3 l = ocn + (bp / aa) * rdn;
+        const float h = aa * (1.f - dot3(l, l));
+        if (h >= 0.f) {
+          const float cc = dot3(ocn, ocn) - 1.f;
+          const float qq = bp + sgnf(bp) * sqrtf(h);
+          const float r0 = cc / qq, r1 = qq / aa;
+          const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
+          // intersection program (shaders.slang:96-106) + any-hit insertion of entry and exit (:138-159)
+          if (!(t1 < cur_t)) {
+            if (cur_t < t0) {
+              if (t0 <= A.tmax) {
+                kbuf_insert<K>(bt, bi, t0, 2u * s + 1u);
+                kbuf_insert<K>(bt, bi, t1, 2u * s);
+              }
+            } else if (cur_t < t1 && t1 <= A.tmax) {
+              kbuf_insert<K>(bt, bi, t1, 2u * s);
+            }
+          }
+        }
+      }
+      if (sp == 0) break;
+      --sp;
+      node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
+    }
+    // ---- integrate the gathered events in order (shaders.slang:210-226) ----
+    bool end = false, stop = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!stop && bt[k] > kEndT) {
+        end = true;
+        stop = true;
+      }
+      if (!stop) {
+        const uint32_t s = bi[k] >> 1, face = bi[k] & 1u;
+        const float4 Dr = A.rec[4 * (int64_t)s + 3];
+        const float hgt = face ? Dr.x : -Dr.x;
+        const float t = bt[k];
+        // update (spline-machine.slang:103-138)
+        const float dt = fmaxf(t - t_state, 0.f);
+        const float Df = (float)D0;
+        const float area = fmaxf(Df * dt, 0.f);
+        const float rn0 = safe_div((float)D1, Df), rn1 = safe_div((float)D2, Df), rn2 = safe_div((float)D3, Df);
+        const float e2 = expf(-(float)logT);
+        const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
+        logT = fmax((double)area + logT, 0.0);
+        C0 += w * rn0;
+        C1 += w * rn1;
+        C2 += w * rn2;
+        const float m = ladder((t + t_state) * (0.5f * kPreMulti));
+        dpx += 2.f * w * m * csx;
+        dpy += 2.f * w * csy;
+        csx += w;
+        csy += w * m;
+        D0 += (double)hgt;
+        D1 += (double)(hgt * Dr.y);
+        D2 += (double)(hgt * Dr.z);
+        D3 += (double)(hgt * Dr.w);
+        t_state = t;
+        const uint32_t orig = A.perm[s];
+        if (A.tri) A.tri[r + (int64_t)iter * A.nr] = (int32_t)(2u * orig + face);
+        if (A.touch) atomicAdd(A.touch + orig, 1);
+        ++iter;
+        if (!(logT < (double)kLogCutoff && iter < A.max_iters)) stop = true;
+      }
+    }
+    if (end) break;
+  }
+
+  // extract_color (spline-machine.slang:146-153) + distortion assembly (fast_ellipsoid_splinetracer.py:81-85)
+  float* oc = A.color + 5 * r;
+  oc[0] = C0;
+  oc[1] = C1;
+  oc[2] = C2;
+  oc[3] = (float)(-expm1(-logT));
+  oc[4] = dpx - dpy;
+  float4* os = reinterpret_cast<float4*>(A.states + 16 * r);
+  os[0] = make_float4(dpx, dpy, csx, csy);
+  os[1] = make_float4(0.f, 0.f, 0.f, t_state);
+  os[2] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
+  os[3] = make_float4((float)logT, C0, C1, C2);
+  A.iters[r] = iter;
+  double* s64 = A.st64 + 6 * r;
+  s64[0] = D0;
+  s64[1] = D1;
+  s64[2] = D2;
+  s64[3] = D3;
+  s64[4] = logT;
+  s64[5] = 0.0;
+}
+
+hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
+                          int32_t* spill, hipStream_t st) {
+  SceneLayout L = scene_layout(P.num_prims);
+  uint32_t* header = at<uint32_t>(ws, L.header);
+  int32_t* cand = at<int32_t>(ws, L.cand);
+  hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, st, header);
+  if (O.touch_count && P.num_prims > 0)
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, O.touch_count, P.num_prims);
+  if (R.num_rays <= 0) return hipGetLastError();
+  if (P.num_prims > 0)
+    hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, at<float>(ws, L.aabb),
+                       P.num_prims, R.origins, R.tmin, cand, header);
+  FwdArgs A;
+  A.rec = at<float4>(ws, L.rec);
+  A.nodes = at<float4>(ws, L.nodes);
+  A.perm = at<uint32_t>(ws, L.vals0);
+  A.colr = at<float4>(ws, L.colr);
+  A.cand = cand;
+  A.header = header;
+  A.means = P.means;
+  A.scales = P.scales;
+  A.quats = P.quats;
+  A.dens = P.densities;
+  A.feat = P.features;
+  A.color_mode = P.color_mode;
+  A.n = P.num_prims;
+  A.rayo = R.origins;
+  A.rayd = R.directions;
+  A.nr = R.num_rays;
+  A.tmin = R.tmin;
+  A.tmax = R.tmax;
+  A.max_iters = R.max_iters;
+  A.color = O.color;
+  A.states = O.states;
+  A.iters = O.iters;
+  A.tri = O.tri_collection;
+  A.touch = O.touch_count;
+  A.init_drgb = O.initial_drgb;
+  A.st64 = O.ray_state64;
+  A.spill = spill;
+  dim3 grid(grid_for(R.num_rays, kFwdBlock));
+  if (P.num_prims <= 0) {
+    // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0
+    A.max_iters = 0;
+  }
+  switch (R.kbuf) {
+    case 8: hipLaunchKernelGGL(k_trace_forward<8>, grid, dim3(kFwdBlock), 0, st, A); break;
+    case 32: hipLaunchKernelGGL(k_trace_forward<32>, grid, dim3(kFwdBlock), 0, st, A); break;
+    default: hipLaunchKernelGGL(k_trace_forward<16>, grid, dim3(kFwdBlock), 0, st, A); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ever

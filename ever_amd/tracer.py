@@ -1,0 +1,171 @@
+"""Host-side driver of the HIP tracer: input validation, buffer allocation (PyTorch), C-ABI calls.
+
+Mirrors the reference's pybind layer (splinetracers/fast_ellipsoid_splinetracer/py_binding.cpp):
+input checks :33-61 and :104-116, output allocation :218-235, saved-for-backward tensors :151-193.
+All buffers are PyTorch tensors on the tracer's device; every call is enqueued on the current stream and
+nothing here synchronizes with the host.
+"""
+import ctypes
+import os
+
+import torch
+
+from . import _capi as C
+
+BUFFER_SIZE = 16  # events gathered per traversal (shaders.slang:18)
+_KBUF = int(os.environ.get("EVER_KBUF", BUFFER_SIZE))
+
+
+def set_kbuf(k):
+    """Select the k-buffer size (8, 16 or 32) used by subsequent traces."""
+    global _KBUF
+    if k not in (8, 16, 32):
+        raise ValueError("kbuf must be 8, 16 or 32")
+    _KBUF = k
+
+
+def get_kbuf():
+    return _KBUF
+
+
+def _check_float(x, name, last=None, device=None):
+    # py_binding.cpp:33-61 (CHECK_CUDA, CHECK_CONTIGUOUS, CHECK_FLOAT, last dimension)
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name} must be a tensor")
+    if not x.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if device is not None and x.device != device:
+        raise RuntimeError(f"{name} must be on the same device")
+    if not x.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+    if x.dtype != torch.float32:
+        raise RuntimeError(f"{name} must have float32 type")
+    if last is not None and x.size(-1) != last:
+        raise RuntimeError(f"{name} must have last dimension with size {last}")
+
+
+class Scene:
+    """Primitive set + LBVH workspace (replaces Primitives + GAS, py_binding.cpp:94-149)."""
+
+    def __init__(self, means, scales, quats, densities, features, color_mode=C.EVER_COLOR_DC, sh_origin=None,
+                 sh_degree=0, activation=0):
+        device = means.device
+        n = means.shape[0]
+        _check_float(means, "means", 3, device)
+        _check_float(scales, "scales", 3, device)
+        _check_float(quats, "quats", 4, device)
+        _check_float(features, "colors", 3, device)
+        _check_float(densities, "densities", None, device)
+        if scales.size(0) != n:
+            raise RuntimeError("All inputs (scales) must have the same 0 dimension")
+        if quats.size(0) != n:
+            raise RuntimeError("All inputs (quats) must have the same 0 dimension")
+        if features.size(0) != n:
+            raise RuntimeError("All inputs (colors) must have the same 0 dimension")
+        if densities.size(0) != n:
+            raise RuntimeError("All inputs (densities) must have the same 0 dimension")
+        if features.dim() != 3 or features.size(2) != 3:
+            raise RuntimeError("Features must have 3 channels. (N, d, 3)")
+        if densities.numel() != n:
+            raise RuntimeError("densities must hold one value per primitive")
+        self.device = device
+        self.n = n
+        self.tensors = (means, scales, quats, densities, features)
+        self.feature_size = features.size(1)
+        self.color_mode = color_mode
+        self.sh_degree = sh_degree
+        self.activation = activation
+        self.sh_origin = [0.0, 0.0, 0.0] if sh_origin is None else [float(v) for v in sh_origin]
+        self.prims = C.Prims(C.ptr(means), C.ptr(scales), C.ptr(quats), C.ptr(densities), C.ptr(features), n,
+                             self.feature_size, color_mode, sh_degree, activation,
+                             (ctypes.c_float * 3)(*self.sh_origin), 0.0)
+        L = C.lib()
+        self.ws_bytes = L.ever_scene_workspace_bytes(n)
+        self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
+
+    def build(self):
+        L = C.lib()
+        with torch.cuda.device(self.device):
+            C.check(L.ever_build_scene(ctypes.byref(self.prims), C.ptr(self.workspace), self.ws_bytes,
+                                       C.stream_ptr()), "ever_build_scene")
+        return self
+
+
+class SavedForBackward:
+    """Mirror of fesSavedForBackward (py_binding.cpp:151-193)."""
+
+    def __init__(self, states, diracs, faces, touch_count, iters):
+        self.states = states
+        self.diracs = diracs
+        self.faces = faces
+        self.touch_count = touch_count
+        self.iters = iters
+
+
+def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None):
+    """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors."""
+    _check_float(rayo, "ray_origins", 3, scene.device)
+    _check_float(rayd, "ray_directions", 3, scene.device)
+    nr = rayo.numel() // 3
+    if rayd.numel() != 3 * nr:
+        raise RuntimeError("ray_origins and ray_directions must hold the same number of rays")
+    if max_iters < 0:
+        raise RuntimeError("max_iters must be >= 0")
+    dev = scene.device
+    L = C.lib()
+    color = torch.empty((nr, 5), dtype=torch.float32, device=dev)
+    states = torch.empty((nr, 16), dtype=torch.float32, device=dev)
+    iters = torch.empty((nr,), dtype=torch.int32, device=dev)
+    alloc = torch.zeros if zero_tail else torch.empty
+    tri = alloc((nr * max_iters,), dtype=torch.int32, device=dev) if store_events else None
+    touch = torch.empty((scene.n,), dtype=torch.int32, device=dev)
+    init = torch.empty((nr, 4), dtype=torch.float32, device=dev)
+    st64 = torch.empty((nr, 6), dtype=torch.float64, device=dev)
+    fws_bytes = L.ever_forward_workspace_bytes(nr)
+    fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
+    out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
+                       C.ptr(st64), C.ptr(fws), fws_bytes)
+    with torch.cuda.device(dev):
+        C.check(L.ever_trace_forward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
+                                     ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
+    return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
+                ray_state64=st64, rays=rays)
+
+
+def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
+    """Adjoint pass (SplineTracer.backward, fast_ellipsoid_splinetracer.py:105-219). Returns a dict of grads."""
+    dev = scene.device
+    n = scene.n
+    nr = rayo.numel() // 3
+    L = C.lib()
+    means, scales, quats, densities, features = scene.tensors
+    g = grad_out.contiguous().float()
+    out = dict(
+        dmeans=torch.empty((n, 3), dtype=torch.float32, device=dev),
+        dscales=torch.empty((n, 3), dtype=torch.float32, device=dev),
+        dquats=torch.empty((n, 4), dtype=torch.float32, device=dev),
+        ddensities=torch.empty((n,), dtype=torch.float32, device=dev),
+        dfeatures=torch.empty(features.shape, dtype=torch.float32, device=dev),
+        drayo=torch.empty((nr, 3), dtype=torch.float32, device=dev),
+        drayd=torch.empty((nr, 3), dtype=torch.float32, device=dev),
+        dinitial_drgb=torch.empty((nr, 4), dtype=torch.float32, device=dev),
+    )
+    nw = 0
+    if wcts is not None:
+        wcts = wcts.contiguous().float()
+        _check_float(wcts, "wcts", 4, dev)
+        nw = wcts.numel() // 16
+        out["dmeans2D"] = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
+    io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
+                      C.ptr(fwd["ray_state64"]), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
+                      C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),
+                      C.ptr(out["drayd"]), C.ptr(out.get("dmeans2D")), C.ptr(out["dinitial_drgb"]), C.ptr(ws),
+                      ws_bytes)
+    with torch.cuda.device(dev):
+        C.check(L.ever_trace_backward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
+                                      ctypes.byref(rays), ctypes.byref(io), C.stream_ptr()), "ever_trace_backward")
+    return out

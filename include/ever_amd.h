@@ -1,0 +1,134 @@
+/*
+ * ever_amd — C ABI of the MI355X (gfx950) exact volumetric ellipsoid tracer.
+ *
+ * Plain C, plain pointers and sizes; every device pointer is HBM memory owned by the caller (PyTorch
+ * tensors in the Python binding). Every entry point enqueues on the given hipStream_t (passed as void*)
+ * and never synchronizes the host, allocates memory or copies device data to the host, so a call sequence
+ * can be captured into a hipGraph. Errors are returned as status codes; ever_last_error() gives the
+ * message of the last failing call on the calling thread.
+ *
+ * The entry points replace these reference interfaces (paths relative to the reference repository root):
+ *   ever_build_scene      splinetracers/fast_ellipsoid_splinetracer/py_binding.cpp:99-132 (Primitives.add_primitives)
+ *                         + create_aabbs.cu:117-132 + GAS.cpp:56-153 (GAS::build, OptiX BVH -> LBVH)
+ *                         + splinetracers/slang/sh_kernel.slang:19-42 (per-primitive SH colour, color_mode 1)
+ *   ever_trace_forward    py_binding.cpp:210-258 (Forward.trace_rays) -> Forward.cpp:23-82,
+ *                         initialize_density.cu:152-208, slang/shaders.slang:171-235
+ *   ever_trace_backward   splinetracers/fast_ellipsoid_splinetracer.py:105-219 (SplineTracer.backward) ->
+ *                         slang/backwards_kernel.slang:259-367 (backwards_kernel), :374-428
+ *                         (backwards_initial_drgb_kernel), gradient clamps fast_ellipsoid_splinetracer.py:201-211
+ *   ever_eval_sh          eval_sh.py:31-55 (EvalSH.forward) -> splinetracers/slang/sh_kernel.slang:19-42
+ *   ever_eval_sh_backward eval_sh.py:57-68 (EvalSH.backward) -> splinetracers/slang/sh_kernel.slang:45-98
+ */
+#ifndef EVER_AMD_H
+#define EVER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EVER_ABI_VERSION 1
+
+/* status codes */
+#define EVER_OK 0
+#define EVER_ERR_INVALID 1   /* bad argument (shape, null pointer, size) */
+#define EVER_ERR_HIP 2       /* a HIP runtime call or kernel launch failed */
+#define EVER_ERR_WORKSPACE 3 /* workspace smaller than the *_workspace_bytes() query */
+
+/* colour modes */
+#define EVER_COLOR_DC 0 /* reference: colour = SH_C0 * features[i,0,:] + 0.5 (shaders.slang:65-82) */
+#define EVER_COLOR_SH 1 /* colour = SH(features[i], normalize(mean_i - sh_origin)), fused prepass */
+
+/* Primitive set. Layout of each array as in the reference's Primitives (structs.h:56-68). */
+typedef struct ever_prims {
+  const float* means;     /* (N,3) */
+  const float* scales;    /* (N,3) */
+  const float* quats;     /* (N,4), (w,x,y,z) */
+  const float* densities; /* (N) */
+  const float* features;  /* (N,d,3) SH coefficients */
+  int64_t num_prims;
+  int32_t feature_size; /* d = (deg+1)^2 */
+  int32_t color_mode;   /* EVER_COLOR_* */
+  int32_t sh_degree;    /* color_mode EVER_COLOR_SH only */
+  int32_t activation;   /* color_mode EVER_COLOR_SH only: 0 none, 1 softplus(beta=10) */
+  float sh_origin[3];   /* color_mode EVER_COLOR_SH only */
+  float reserved;
+} ever_prims;
+
+/* Ray batch and trace options (Forward::trace_rays arguments, Forward.cpp:23-82). */
+typedef struct ever_rays {
+  const float* origins;    /* (R,3) */
+  const float* directions; /* (R,3), need not be unit */
+  int64_t num_rays;
+  float tmin;
+  float tmax;
+  int32_t max_iters;
+  int32_t kbuf; /* events gathered per traversal: 8, 16 (reference) or 32 */
+} ever_rays;
+
+typedef struct ever_forward_out {
+  float* color;            /* (R,5): R,G,B, opacity, distortion */
+  float* states;           /* (R,16) final SplineState (structs.h:41-54) */
+  int32_t* iters;          /* (R) events processed */
+  int32_t* tri_collection; /* (R*max_iters), event id 2*prim+entry at [ray + k*R]; nullable */
+  int32_t* touch_count;    /* (N) events per primitive (exact); nullable; zeroed by the call */
+  float* initial_drgb;     /* (R,4) density/colour already open at the ray start */
+  double* ray_state64;     /* (R,6) FP64 final accumulators, read by ever_trace_backward */
+  void* workspace;         /* traversal-stack spill, ever_forward_workspace_bytes(R) bytes */
+  size_t workspace_bytes;
+} ever_forward_out;
+
+typedef struct ever_backward_io {
+  const float* grad_out;      /* (R,5) */
+  const int32_t* iters;       /* forward outputs */
+  const int32_t* tri_collection;
+  const float* states;
+  const double* ray_state64;
+  const float* wcts; /* (W,4,4) world-to-camera matrices for the means2D statistic; nullable */
+  int64_t num_wcts;
+  float* dmeans;     /* (N,3) */
+  float* dscales;    /* (N,3) */
+  float* dquats;     /* (N,4) */
+  float* ddensities; /* (N) */
+  float* dfeatures;  /* (N,d,3) */
+  float* drayo;      /* (R,3) */
+  float* drayd;      /* (R,3) */
+  float* dmeans2D;   /* (N,2); written only when wcts != NULL */
+  float* dinitial_drgb; /* (R,4) adjoint of the start-inside density; nullable */
+  void* workspace;      /* ever_backward_workspace_bytes(N, R, W) bytes */
+  size_t workspace_bytes;
+} ever_backward_io;
+
+int ever_abi_version(void);
+/* sizeof() of each ABI struct, in the order ever_prims, ever_rays, ever_forward_out, ever_backward_io. */
+int ever_struct_sizes(int64_t* out4);
+const char* ever_last_error(void);
+
+/* Scene workspace: LBVH nodes, sort buffers, per-primitive records. Lives from build to backward. */
+size_t ever_scene_workspace_bytes(int64_t num_prims);
+int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The scene workspace is also written: the start-inside candidate list of this ray batch. */
+size_t ever_forward_workspace_bytes(int64_t num_rays);
+int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspace_bytes,
+                       const ever_rays* rays, const ever_forward_out* out, void* stream);
+
+size_t ever_backward_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts);
+int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes,
+                        const ever_rays* rays, const ever_backward_io* io, void* stream);
+
+/* Per-primitive SH colour at normalize(mean - origin), unclamped (sh_kernel.slang:19-42). */
+int ever_eval_sh(const float* means, const float* features, const float* origin, int64_t num_prims,
+                 int32_t feature_size, int32_t sh_degree, float* out_rgb, void* stream);
+/* dL/dfeatures of ever_eval_sh (sh_kernel.slang:45-98); dfeatures is overwritten. */
+int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,
+                          int64_t num_prims, int32_t feature_size, int32_t sh_degree, float* dfeatures,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EVER_AMD_H */
