@@ -1,0 +1,180 @@
+"""Benchmark: Mrays/s of forward + backward of the exact ellipsoid tracer, 1M clustered ellipsoids @ 1920x1080.
+
+Workload (one step): LBVH rebuilt from the primitives, SH degree 3 colour prepass fused into the build, forward
+trace of 1920x1080 rays, backward with N(0,1) upstream gradients on all five output columns, through the
+drop-in autograd path (trace_rays_sh + .backward()). Scene: 1,000,000 ellipsoids from a 64-component Gaussian
+mixture (std 0.15) in [-2,2]^3, aspect ratios up to 100:1, densities logU[0.1,200]; camera at radius 4.
+
+Prints one JSON line: value = rays / second of whole steps (Mrays/s), the roofline record of the dominant
+kernel (algorithmic bytes / its event-timed duration vs 8 TB/s HBM) and the CPU baseline (the FP32 C++
+restatement of the reference algorithm, OpenMP, timed on a bounded ray subsample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s fwd+bwd, 1M ellipsoids @1080p, plain vs proposal at ≤1e-4 color err"
+HBM_PEAK_GBS = 8000.0
+P_BYTES = 56  # per ray-primitive interaction record: mean 12 + scale 12 + quat 16 + density 4 + colour 12
+G_BYTES = 56  # gradient record of the same fields
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--prims", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--kbuf", type=int, default=16)
+    ap.add_argument("--max-iters", type=int, default=500)
+    ap.add_argument("--cpu-sample", type=int, default=1024, help="rays traced by the CPU baseline (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(out, nr, n, d, max_iters):
+    """Forward and backward algorithmic bytes of one step (ray I/O, primitive records per distinct primitive
+    a ray touches, stored event ids, saved state, SH prepass). Distinct primitives per ray are bounded below
+    by max(#entry events, #exit events)."""
+    import torch
+    iters = out["iters"].long()
+    tri = out["tri_collection"].view(max_iters, nr)
+    E = int(iters.sum())
+    ent = torch.zeros(nr, dtype=torch.long, device=iters.device)
+    ext = torch.zeros(nr, dtype=torch.long, device=iters.device)
+    kmax = int(iters.max()) if nr > 0 else 0
+    for k0 in range(0, kmax, 64):
+        k1 = min(kmax, k0 + 64)
+        blk = tri[k0:k1]
+        valid = torch.arange(k0, k1, device=iters.device).view(-1, 1) < iters.view(1, -1)
+        face = (blk & 1).bool()
+        ent += (valid & face).sum(0)
+        ext += (valid & ~face).sum(0)
+    U = int(torch.maximum(ent, ext).sum())
+    sh_fwd = n * (12 + 12 * d + 16)
+    fwd = nr * 44 + U * P_BYTES + nr * 68 + 4 * E + sh_fwd
+    bwd = nr * (44 + 68) + 4 * E + U * (P_BYTES + G_BYTES) + n * (12 + 12 * d + 12 + 12 * d)
+    return dict(fwd=fwd, bwd=bwd, events=E, distinct=U)
+
+
+def cpu_baseline(sc, o, d, eye, sample, max_iters, nr_full):
+    """FP32 C++ restatement of the reference (oracle, OpenMP, CPU BVH) on a deterministic ray subsample;
+    the per-ray time is extrapolated to the full image and the BVH build counted once."""
+    import numpy as np
+
+    from oracle import oracle
+    from oracle import quadrature as Q
+    stride = max(1, o.shape[0] // sample)
+    idx = np.arange(0, o.shape[0], stride)[:sample]
+    means = sc["means"].numpy()
+    t0 = time.time()
+    rgb = oracle.eval_sh(means, sc["features"].numpy(), 3, np.asarray(eye))  # per-primitive SH prepass
+    rgb = np.logaddexp(0.0, 10.0 * rgb) / 10.0                                # softplus(beta=10)
+    feats = Q.RGB2SH(rgb).reshape(-1, 1, 3).astype(np.float32)
+    t_sh = time.time() - t0
+    args = [means, sc["scales"].numpy(), sc["quats"].numpy(), sc["densities"].numpy(), feats]
+    ro, rd = o.numpy()[idx], d.numpy()[idx]
+    t0 = time.time()
+    oracle.forward(*args, ro[:1], rd[:1], 0.0, 100.0, max_iters=max_iters, use_bvh=True, dtype=np.float32)
+    t_build = time.time() - t0
+    t0 = time.time()
+    f = oracle.forward(*args, ro, rd, 0.0, 100.0, max_iters=max_iters, use_bvh=True, dtype=np.float32)
+    t_fwd = time.time() - t0 - t_build
+    g = np.random.default_rng(1).normal(size=(len(idx), 5)).astype(np.float32)
+    t0 = time.time()
+    oracle.backward(*args, ro, rd, 0.0, max_iters, g, f, dtype=np.float32)
+    t_bwd = time.time() - t0
+    per_ray = max(t_fwd, 0.0) / len(idx) + t_bwd / len(idx)
+    total = t_sh + t_build + per_ray * nr_full
+    return dict(value=round(nr_full / total / 1e6, 6), unit="Mrays/s", cores=oracle.num_threads(), kind="port",
+                sample=(f"{len(idx)} of {nr_full} rays (every {stride}th pixel), FP32, CPU BVH build "
+                        f"{t_build:.2f}s + SH prepass {t_sh:.2f}s counted once, per-ray fwd+bwd "
+                        f"{1e3 * per_ray:.3f} ms extrapolated to the full image"))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the tracer shards by rays, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from ever_amd import scenes
+    from ever_amd import tracer as T
+    from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
+
+    dev = torch.device("cuda:0")
+    T.set_kbuf(a.kbuf)
+    sc, o, d = scenes.config3(width=a.width, height=a.height, n=a.prims)
+    eye = (0.0, 0.0, -4.0)
+    nr = o.shape[0]
+    D = {k: v.to(dev).contiguous() for k, v in sc.items()}
+    params = [D[k].requires_grad_(True) for k in ("means", "scales", "quats", "densities", "features")]
+    og, dg = o.to(dev), d.to(dev)
+    grad = scenes.upstream_grad(nr, seed=1).to(dev)
+
+    def step():
+        out = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters)
+        torch.autograd.backward(out, grad)
+        for p in params:
+            p.grad = None
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = T.StageTimer()
+    T.stage_timer = timer
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    T.stage_timer = None
+    stages = timer.summary()
+
+    # one untimed forward for the algorithmic byte count of this workload
+    scene = T.Scene(*[x.detach() for x in params], color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
+    out = T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters)
+    ab = algorithmic_bytes(out, nr, a.prims, 16, a.max_iters)
+    fwd_ms, bwd_ms = stages.get("forward", 0.0), stages.get("backward", 0.0)
+    dom, dom_ms, dom_bytes = ("forward", fwd_ms, ab["fwd"]) if fwd_ms >= bwd_ms else ("backward", bwd_ms, ab["bwd"])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    rec = {
+        "metric": METRIC,
+        "value": round(nr * a.steps / dt / 1e6, 4),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * dt / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1M clustered ellipsoids (64-GMM, aspect<=100, SH3) @1920x1080, fwd+bwd, LBVH "
+                               "rebuilt per step, plain traversal", "prims": a.prims, "rays": nr, "kbuf": a.kbuf,
+                   "max_iters": a.max_iters, "sh_degree": 3},
+        "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "events_per_ray": round(ab["events"] / nr, 3),
+        "roofline": {"bound": "hbm", "kernel": f"k_trace_{dom} (+ its small prologue kernels)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": int(dom_bytes)},
+    }
+    if not a.no_cpu and a.cpu_sample > 0:
+        rec["cpu_baseline"] = cpu_baseline(sc, o, d, eye, a.cpu_sample, a.max_iters, nr)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

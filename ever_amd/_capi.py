@@ -48,6 +48,7 @@ SIGNATURES = {
     "ever_abi_version": ([], ctypes.c_int),
     "ever_struct_sizes": ([ctypes.POINTER(_i64)], ctypes.c_int),
     "ever_last_error": ([], ctypes.c_char_p),
+    "ever_scene_layout": ([_i64, ctypes.POINTER(_i64)], ctypes.c_int),
     "ever_scene_workspace_bytes": ([_i64], ctypes.c_size_t),
     "ever_build_scene": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "ever_forward_workspace_bytes": ([_i64], ctypes.c_size_t),
@@ -99,3 +100,14 @@ def ptr(t):
 
 def stream_ptr(device=None):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+LAYOUT_NAMES = ("header", "aabb", "keys0", "vals0", "keys1", "vals1", "hist", "nodes", "parent", "leaf_parent",
+                "flags", "rec", "colr", "cand", "total")
+
+
+def scene_layout(n):
+    """Byte offsets of the scene workspace regions (ever_scene_layout)."""
+    o = (_i64 * 15)()
+    check(lib().ever_scene_layout(n, o), "ever_scene_layout")
+    return dict(zip(LAYOUT_NAMES, list(o)))

@@ -64,6 +64,15 @@ int ever_struct_sizes(int64_t* out4) {
 
 const char* ever_last_error(void) { return g_last_error.c_str(); }
 
+int ever_scene_layout(int64_t num_prims, int64_t* o) {
+  if (!o) return fail(EVER_ERR_INVALID, "offsets is null");
+  ever::SceneLayout L = ever::scene_layout(num_prims > 0 ? num_prims : 0);
+  const size_t v[15] = {L.header, L.aabb, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes,
+                        L.parent, L.leaf_parent, L.flags, L.rec, L.colr, L.cand, L.total};
+  for (int i = 0; i < 15; ++i) o[i] = (int64_t)v[i];
+  return EVER_OK;
+}
+
 size_t ever_scene_workspace_bytes(int64_t num_prims) {
   return ever::scene_layout(num_prims > 0 ? num_prims : 0).total;
 }

@@ -163,7 +163,9 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
     // ---- traversal: gather the K nearest events beyond cur_t ----
     int sp = 0;
     int32_t node = root;
-    while (true) {
+    int64_t steps = 0;
+    const int64_t max_steps = 2 * A.n + 2;  // a tree visits each node at most once per traversal
+    while (++steps <= max_steps) {
       if (node >= 0) {
         const float4* nd = A.nodes + 4 * (int64_t)node;
         const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         const bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, cur_t, t_hi, tn0);
         const bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, cur_t, t_hi, tn1);
         const int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-        if (h0 && h1) {
+        if (h0 && h1 && sp < kLdsDepth + kSpillDepth) {  // depth of a 30-bit LBVH never exceeds 62
           const bool first0 = tn0 <= tn1;
           const int32_t farc = first0 ? c1 : c0;
           if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = farc;

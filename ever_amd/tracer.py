@@ -16,6 +16,47 @@ BUFFER_SIZE = 16  # events gathered per traversal (shaders.slang:18)
 _KBUF = int(os.environ.get("EVER_KBUF", BUFFER_SIZE))
 
 
+class StageTimer:
+    """Records a pair of torch.cuda.Events around each library call (on the stream the kernels run on).
+
+    Enabled by assigning an instance to ``tracer.stage_timer``; ``summary()`` returns mean milliseconds per
+    stage after a synchronize.
+    """
+
+    def __init__(self):
+        self.events = {}
+
+    def start(self, name):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        return name, e0
+
+    def stop(self, token):
+        name, e0 = token
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.events.setdefault(name, []).append((e0, e1))
+
+    def summary(self):
+        torch.cuda.synchronize()
+        return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self.events.items()}
+
+    def reset(self):
+        self.events = {}
+
+
+stage_timer = None
+
+
+def _t0(name):
+    return stage_timer.start(name) if stage_timer is not None else None
+
+
+def _t1(tok):
+    if tok is not None:
+        stage_timer.stop(tok)
+
+
 def set_kbuf(k):
     """Select the k-buffer size (8, 16 or 32) used by subsequent traces."""
     global _KBUF
@@ -86,8 +127,10 @@ class Scene:
     def build(self):
         L = C.lib()
         with torch.cuda.device(self.device):
+            tok = _t0("build")
             C.check(L.ever_build_scene(ctypes.byref(self.prims), C.ptr(self.workspace), self.ws_bytes,
                                        C.stream_ptr()), "ever_build_scene")
+            _t1(tok)
         return self
 
 
@@ -127,8 +170,10 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
                        C.ptr(st64), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
+        tok = _t0("forward")
         C.check(L.ever_trace_forward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
                                      ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
+        _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
                 ray_state64=st64, rays=rays)
 
@@ -166,6 +211,8 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
                       C.ptr(out["drayd"]), C.ptr(out.get("dmeans2D")), C.ptr(out["dinitial_drgb"]), C.ptr(ws),
                       ws_bytes)
     with torch.cuda.device(dev):
+        tok = _t0("backward")
         C.check(L.ever_trace_backward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
                                       ctypes.byref(rays), ctypes.byref(io), C.stream_ptr()), "ever_trace_backward")
+        _t1(tok)
     return out

@@ -106,6 +106,10 @@ int ever_abi_version(void);
 int ever_struct_sizes(int64_t* out4);
 const char* ever_last_error(void);
 
+/* Byte offsets of the scene workspace regions, for inspection by tests (order: header, aabb, keys0, vals0,
+ * keys1, vals1, hist, nodes, parent, leaf_parent, flags, rec, colr, cand, total). */
+int ever_scene_layout(int64_t num_prims, int64_t* offsets15);
+
 /* Scene workspace: LBVH nodes, sort buffers, per-primitive records. Lives from build to backward. */
 size_t ever_scene_workspace_bytes(int64_t num_prims);
 int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream);

@@ -1,0 +1,74 @@
+"""The C-ABI library loads, exports every function include/ever_amd.h declares, and validates arguments on the
+host (no GPU work is issued by these calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from ever_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "ever_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(ever_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_every_header_function_is_exported_and_bound():
+    names = header_functions()
+    assert len(names) >= 12
+    L = C.lib()
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in C.SIGNATURES, n
+
+
+def test_abi_version_and_struct_sizes():
+    L = C.lib()
+    assert L.ever_abi_version() == 1
+    sizes = (ctypes.c_int64 * 4)()
+    assert L.ever_struct_sizes(sizes) == 0
+    assert list(sizes) == [ctypes.sizeof(C.Prims), ctypes.sizeof(C.Rays), ctypes.sizeof(C.ForwardOut),
+                           ctypes.sizeof(C.BackwardIO)]
+
+
+def test_workspace_queries_grow_with_size():
+    L = C.lib()
+    a, b = L.ever_scene_workspace_bytes(1000), L.ever_scene_workspace_bytes(100000)
+    assert 0 < a < b
+    assert L.ever_scene_workspace_bytes(100000) >= 100000 * 64 * 2
+    assert L.ever_forward_workspace_bytes(10) > 0
+    lay = C.scene_layout(1000)
+    offs = [lay[k] for k in C.LAYOUT_NAMES]
+    assert offs == sorted(offs) and lay["total"] == a
+    assert all(o % 256 == 0 for o in offs)
+
+
+def test_invalid_arguments_return_status_codes():
+    L = C.lib()
+    assert L.ever_build_scene(None, None, 0, None) == 1
+    assert b"prims is null" in L.ever_last_error()
+    p = C.Prims()
+    p.num_prims = 10
+    p.feature_size = 1
+    assert L.ever_build_scene(ctypes.byref(p), None, 0, None) == 1  # null arrays
+    p.num_prims = 0
+    assert L.ever_build_scene(ctypes.byref(p), None, 0, None) == 1  # null workspace
+    buf = ctypes.create_string_buffer(16)
+    assert L.ever_build_scene(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 0, None) == 3
+    r = C.Rays()
+    r.kbuf = 7
+    assert L.ever_trace_forward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r), None,
+                                None) == 1
+    assert b"kbuf" in L.ever_last_error()
+    assert L.ever_eval_sh(None, None, None, 5, 1, 4, None, None) == 1
+
+
+def test_python_layer_validates_like_the_reference():
+    import torch
+    from ever_amd import tracer as T
+    with pytest.raises(RuntimeError, match="CUDA"):
+        T.Scene(torch.zeros(2, 3), torch.zeros(2, 3), torch.zeros(2, 4), torch.zeros(2), torch.zeros(2, 1, 3))
