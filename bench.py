@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--kbuf", type=int, default=16)
     ap.add_argument("--max-iters", type=int, default=500)
-    ap.add_argument("--cpu-sample", type=int, default=1024, help="rays traced by the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=131072, help="rays traced by the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 

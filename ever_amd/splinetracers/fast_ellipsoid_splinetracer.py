@@ -17,8 +17,8 @@ from typing import Any
 import torch
 from torch.autograd import Function
 
-from ... import _capi as C
-from ... import tracer as T
+from .. import _capi as C
+from .. import tracer as T
 
 # gradient clamps of the reference's backward (fast_ellipsoid_splinetracer.py:201-211) are applied in the
 # library's per-primitive epilogue and in the per-ray adjoint kernel.
