@@ -74,12 +74,12 @@ __global__ void k_invert4(const float* __restrict__ w, int64_t nw, float* __rest
     for (int j = 0; j < 4; ++j) out[16 * k + 4 * i + j] = (float)a[i][j + 4];
 }
 
-// safe root of one event, value only
-__device__ __forceinline__ float event_t(const BwdArgs& A, f3 org, f3 dir, int32_t id) {
+// safe root of one event (FP64), value only
+__device__ __forceinline__ double event_t(const BwdArgs& A, d3 org, f3 dir, int32_t id) {
   const int64_t p = id >> 1;
   SafeRoots S;
   float4 q = reinterpret_cast<const float4*>(A.quats)[p];
-  safe_roots(org - ld3(A.means + 3 * p), dir, ld3(A.scales + 3 * p), q, S);
+  safe_roots(org - tod(ld3(A.means + 3 * p)), dir, ld3(A.scales + 3 * p), q, S);
   return safe_root_select(S, id & 1);
 }
 
@@ -97,7 +97,9 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
     return;
   }
   const f3 dir = ld3(A.rayd + 3 * r);  // raw direction (backwards_kernel.slang:288-289)
-  const f3 org = ld3(A.rayo + 3 * r) + A.tmin * dir;
+  const f3 rof = ld3(A.rayo + 3 * r);
+  const d3 org = mkd((double)rof.x + (double)A.tmin * dir.x, (double)rof.y + (double)A.tmin * dir.y,
+                     (double)rof.z + (double)A.tmin * dir.z);
   const float* g = A.grad_out + 5 * r;
   const float gC0 = g[0], gC1 = g[1], gC2 = g[2], gO = g[3], gD = g[4];
   const float4* st = reinterpret_cast<const float4*>(A.states + 16 * r);
@@ -121,10 +123,10 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   }
 
   int32_t id = A.tri[r + (int64_t)(E - 1) * A.nr];
-  float tk = event_t(A, org, dir, id);
+  double tk = event_t(A, org, dir, id);
   for (int k = E - 1; k >= 0; --k) {
     int32_t id_prev = 0;
-    float tprev = 0.f;
+    double tprev = 0.0;
     if (k > 0) {
       id_prev = A.tri[r + (int64_t)(k - 1) * A.nr];
       tprev = event_t(A, org, dir, id_prev);
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
     // reconstruct the state before event k (inverse of the forward's FP64 accumulation)
     const double Dbf0 = Da0 - (double)hgt, Dbf1 = Da1 - (double)(hgt * cr.y), Dbf2 = Da2 - (double)(hgt * cr.z),
                  Dbf3 = Da3 - (double)(hgt * cr.w);
-    const float dt_raw = tk - tprev;
+    const float dt_raw = (float)(tk - tprev);
     const float dt = fmaxf(dt_raw, 0.f);
     const float Df = (float)Dbf0;
     const float a_raw = Df * dt;
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
     const float e2 = expf(-(float)Lbf);
     const float w_raw = om * e2;
     const float w = clipf(w_raw, 0.f, 1.f);
-    const float mid = (tk + tprev) * (0.5f * kPreMulti);
+    const float mid = (float)((tk + tprev) * (0.5 * kPreMulti));
     const float m = ladder(mid);
     const float csxbf = csxa - w, csybf = csya - w * m;
     const float R1 = (float)Dbf1, R2 = (float)Dbf2, R3 = (float)Dbf3;
@@ -193,11 +195,12 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
     const float4 q = reinterpret_cast<const float4*>(A.quats)[p];
     const f3 mean = ld3(A.means + 3 * p);
     const f3 scl = ld3(A.scales + 3 * p);
-    const f3 ro = org - mean;
+    const d3 rod = org - tod(mean);
+    const f3 ro = tof(rod);
     SafeRoots S;
     f3 rob = mk3(0.f, 0.f, 0.f), rdb = mk3(0.f, 0.f, 0.f), sbar = mk3(0.f, 0.f, 0.f);
     float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (safe_roots(ro, dir, scl, q, S)) safe_roots_bwd(S, ro, dir, scl, q, face, tb, rob, rdb, sbar, qb);
+    if (safe_roots(rod, dir, scl, q, S)) safe_roots_bwd(S, ro, dir, scl, q, face, tb, rob, rdb, sbar, qb);
     orgb = orgb + rob;
     dirb = dirb + rdb;
     float* ac = A.acc + 16 * p;

@@ -140,43 +140,88 @@ __device__ __forceinline__ float softplus10_grad(float x) {
   return bx > 20.f ? 1.f : 1.f / (1.f + expf(-bx));
 }
 
+// FP64 vector helpers for the root evaluation of the backward.
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 mkd(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dotd(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 crossd(d3 a, d3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ d3 tod(f3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ f3 tof(d3 a) { return {(float)a.x, (float)a.y, (float)a.z}; }
+__device__ __forceinline__ double safe_div_d(double a, double b) {
+  const double r = a / (fabs(b) < (double)kTiny ? (double)kTiny : b);
+  return fmax(fmin(r, (double)kMaxVal), (double)kMinVal);
+}
+__device__ __forceinline__ d3 qrot_d(d3 v, float4 q) {
+  d3 u = mkd(-q.y, -q.z, -q.w);
+  d3 t = 2.0 * crossd(u, v);
+  return v + (double)q.x * t + crossd(u, t);
+}
+
 // Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344): the rotated raw
-// direction is l2-normalized, divisions use safe_div. Returns false on a miss.
+// direction is l2-normalized, divisions use safe_div. Evaluated in FP64 (so that 1 - |l|^2 and the root depths
+// keep their precision at grazing incidence and far from the ray origin) and kept in FP32 for the adjoint.
 struct SafeRoots {
   f3 Tro, Trd, rd_rot, ocn, rdn;
-  float a, bp, ba, ll, h, c, sh, sb, qq, r0, r1;
+  float a, bp, ba, omll, h, c, sh, sb, qq, r0, r1;
   f3 l;
+  double t0, t1;  // (min, max) root in FP64
 };
-__device__ __forceinline__ bool safe_roots(f3 ro, f3 rd, f3 s, float4 q, SafeRoots& S) {
-  S.rd_rot = qrot(rd, q);
-  S.Trd = l2n(S.rd_rot);
-  S.Tro = qrot(ro, q);
-  S.ocn = mk3(safe_div(S.Tro.x, s.x), safe_div(S.Tro.y, s.y), safe_div(S.Tro.z, s.z));
-  S.rdn = mk3(safe_div(S.Trd.x, s.x), safe_div(S.Trd.y, s.y), safe_div(S.Trd.z, s.z));
-  S.a = dot3(S.rdn, S.rdn);
-  S.bp = -dot3(S.ocn, S.rdn);
-  S.ba = safe_div(S.bp, S.a);
-  S.l = S.ocn + S.ba * S.rdn;
-  S.ll = dot3(S.l, S.l);
-  S.h = S.a * (1.f - S.ll);
-  S.c = dot3(S.ocn, S.ocn) - 1.f;
-  if (S.h < 0.f) {
+__device__ __forceinline__ bool safe_roots(d3 ro, f3 rd, f3 s, float4 q, SafeRoots& S) {
+  const d3 rd_rot = qrot_d(tod(rd), q);
+  const double nrm = sqrt(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
+  const d3 Trd = mkd(safe_div_d(rd_rot.x, nrm), safe_div_d(rd_rot.y, nrm), safe_div_d(rd_rot.z, nrm));
+  const d3 Tro = qrot_d(ro, q);
+  const d3 ocn = mkd(safe_div_d(Tro.x, s.x), safe_div_d(Tro.y, s.y), safe_div_d(Tro.z, s.z));
+  const d3 rdn = mkd(safe_div_d(Trd.x, s.x), safe_div_d(Trd.y, s.y), safe_div_d(Trd.z, s.z));
+  const double a = dotd(rdn, rdn);
+  const double bp = -dotd(ocn, rdn);
+  const double ba = safe_div_d(bp, a);
+  const d3 l = ocn + ba * rdn;
+  const double omll = 1.0 - dotd(l, l);
+  const double h = a * omll;
+  const double c = dotd(ocn, ocn) - 1.0;
+  S.rd_rot = tof(rd_rot);
+  S.Trd = tof(Trd);
+  S.Tro = tof(Tro);
+  S.ocn = tof(ocn);
+  S.rdn = tof(rdn);
+  S.a = (float)a;
+  S.bp = (float)bp;
+  S.ba = (float)ba;
+  S.l = tof(l);
+  S.omll = (float)omll;
+  S.h = (float)h;
+  S.c = (float)c;
+  if (h < 0.0) {
     S.r0 = S.r1 = -1.f;
+    S.t0 = S.t1 = -1.0;
     return false;
   }
-  S.sh = safe_sqrt(S.h);
-  S.sb = sgnf(S.bp);
-  S.qq = S.bp + S.sb * S.sh;
-  S.r0 = safe_div(S.c, S.qq);
-  S.r1 = safe_div(S.qq, S.a);
+  const double sh = h < (double)kTiny ? 0.0 : sqrt(h);
+  const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
+  const double qq = bp + sb * sh;
+  const double r0 = safe_div_d(c, qq), r1 = safe_div_d(qq, a);
+  S.sh = (float)sh;
+  S.sb = (float)sb;
+  S.qq = (float)qq;
+  S.r0 = (float)r0;
+  S.r1 = (float)r1;
+  S.t0 = fmin(r0, r1);
+  S.t1 = fmax(r0, r1);
   return true;
 }
-__device__ __forceinline__ float safe_root_select(const SafeRoots& S, int face) {
-  return face == 1 ? fminf(S.r0, S.r1) : fmaxf(S.r0, S.r1);
-}
+__device__ __forceinline__ double safe_root_select(const SafeRoots& S, int face) { return face == 1 ? S.t0 : S.t1; }
 // Adjoint of the selected root w.r.t. ro = origin - mean, rd = raw direction, scale and raw quaternion.
 __device__ __forceinline__ void safe_roots_bwd(const SafeRoots& S, f3 ro, f3 rd, f3 s, float4 q, int face,
                                                float tb, f3& rob, f3& rdb, f3& sbar, float4& qb) {
+  // ro enters only through the rotation adjoint, which is linear in it
   float r0b = 0.f, r1b = 0.f;
   if (face == 1) {
     if (S.r0 < S.r1) r0b = tb; else r1b = tb;
@@ -192,7 +237,7 @@ __device__ __forceinline__ void safe_roots_bwd(const SafeRoots& S, f3 ro, f3 rd,
   ab += db;
   bpb = qqb;
   if (!(S.h < kTiny)) hb = qqb * S.sb * 0.5f / S.sh;
-  ab += hb * (1.f - S.ll);
+  ab += hb * S.omll;
   f3 lb = (2.f * (-hb * S.a)) * S.l;
   f3 ocnb = (2.f * cb) * S.ocn + lb;
   float bab = dot3(lb, S.rdn);

@@ -6,8 +6,10 @@
 // Semantics follow the reference's OptiX programs (splinetracers/fast_ellipsoid_splinetracer/slang/shaders.slang):
 //   intersection + report rules :84-107, any-hit k-buffer insertion (strict <) :117-164, raygen loop :171-235,
 //   integrator spline-machine.slang:103-153, start-inside density initialize_density.cu:69-150.
-// Differences, all toward the FP64 oracle: the density/colour accumulator and log-transmittance are kept in
-// FP64 (exactly invertible by the adjoint), 1-exp(-x) is evaluated as -expm1(-x), and touch counts use atomics.
+// Differences, all toward the FP64 oracle: event depths fed to the integrator, the density/colour accumulator
+// and the log-transmittance are FP64 (the accumulators exactly invertible by the adjoint); 1-exp(-x) is
+// evaluated as -expm1(-x); touch counts use atomics; the re-query continues from a (t, event key) cursor
+// instead of t > t_cur, so events of equal depth straddling two k-buffer fills are processed once.
 #include "ever_common.hpp"
 #include "ever_internal.hpp"
 
@@ -72,19 +74,23 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
+// k-buffer entries are 64-bit keys (depth bits << 32 | event key), event key = 2*prim + (exit ? 1 : 0): for
+// depths >= 0 the float bit pattern is monotonic, so one unsigned compare orders by (t, key) with the entry of a
+// primitive before its exit on equal t. Insertion keeps the reference's stable bubble (shaders.slang:146-157).
 template <int K>
-__device__ __forceinline__ void kbuf_insert(float (&bt)[K], uint32_t (&bi)[K], float t, uint32_t id) {
-  if (!(t < bt[K - 1])) return;
+__device__ __forceinline__ void kbuf_insert(uint64_t (&bk)[K], uint64_t v) {
+  if (!(v < bk[K - 1])) return;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    bool sw = t < bt[k];
-    float tt = bt[k];
-    uint32_t ii = bi[k];
-    bt[k] = sw ? t : tt;
-    bi[k] = sw ? id : ii;
-    t = sw ? tt : t;
-    id = sw ? ii : id;
+    const bool sw = v < bk[k];
+    const uint64_t old = bk[k];
+    bk[k] = sw ? v : old;
+    v = sw ? old : v;
   }
+}
+
+__device__ __forceinline__ uint64_t pack_event(float t, uint32_t key) {
+  return ((uint64_t)__float_as_uint(t + 0.f) << 32) | key;
 }
 
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
@@ -96,6 +102,27 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
   float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t_hi));
   t_near = tn;
   return tn <= tf;
+}
+
+// FP64 depth of one event for the integrator: roots of |M (o + t d - mean)| = 1 with the record's M rows and the
+// primitive's mean (the same formula as tri-intersect.slang:268-283, evaluated in double so that depth
+// differences keep their precision far from the origin).
+__device__ __forceinline__ double event_depth_fp64(float4 RA, float4 RB, float4 RC, f3 mu, double ox, double oy,
+                                                   double oz, double dx, double dy, double dz, bool entry) {
+  const double vx = ox - (double)mu.x, vy = oy - (double)mu.y, vz = oz - (double)mu.z;
+  const double o0 = RA.x * vx + RA.y * vy + RA.z * vz, o1 = RB.x * vx + RB.y * vy + RB.z * vz,
+               o2 = RC.x * vx + RC.y * vy + RC.z * vz;
+  const double r0 = RA.x * dx + RA.y * dy + RA.z * dz, r1 = RB.x * dx + RB.y * dy + RB.z * dz,
+               r2 = RC.x * dx + RC.y * dy + RC.z * dz;
+  const double a = r0 * r0 + r1 * r1 + r2 * r2;
+  const double bp = -(o0 * r0 + o1 * r1 + o2 * r2);
+  const double ba = bp / a;
+  const double l0 = o0 + ba * r0, l1 = o1 + ba * r1, l2 = o2 + ba * r2;
+  const double h = fmax(a * (1.0 - (l0 * l0 + l1 * l1 + l2 * l2)), 0.0);
+  const double c = o0 * o0 + o1 * o1 + o2 * o2 - 1.0;
+  const double q = bp + (bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0)) * sqrt(h);
+  const double x0 = c / q, x1 = q / a;
+  return entry ? fmin(x0, x1) : fmax(x0, x1);
 }
 
 template <int K>
@@ -111,6 +138,15 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   const f3 inv = mk3(fabsf(d.x) > 1e-20f ? 1.f / d.x : copysignf(1e20f, d.x),
                      fabsf(d.y) > 1e-20f ? 1.f / d.y : copysignf(1e20f, d.y),
                      fabsf(d.z) > 1e-20f ? 1.f / d.z : copysignf(1e20f, d.z));
+  const double odx = (double)ro.x + (double)A.tmin * rdraw.x, ody = (double)ro.y + (double)A.tmin * rdraw.y,
+               odz = (double)ro.z + (double)A.tmin * rdraw.z;
+  double ddx = rdraw.x, ddy = rdraw.y, ddz = rdraw.z;
+  {
+    const double nn = 1.0 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    ddx *= nn;
+    ddy *= nn;
+    ddz *= nn;
+  }
 
   // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
   double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
@@ -145,22 +181,20 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   }
   reinterpret_cast<float4*>(A.init_drgb)[r] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
 
-  float t_state = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, csx = 0.f, csy = 0.f, dpx = 0.f, dpy = 0.f;
-  double logT = 0.0;
+  float C0 = 0.f, C1 = 0.f, C2 = 0.f, csx = 0.f, csy = 0.f, dpx = 0.f, dpy = 0.f;
+  double t_state = 0.0, logT = 0.0;
+  // cursor over (t, key): initially nothing at t == 0 (the reference requires t > t_cur = 0)
+  float cur_t = 0.f;
+  uint64_t cursor = pack_event(0.f, 0xffffffffu);
   int iter = 0;
   const int32_t root = A.n == 1 ? ~0 : 0;
   int32_t* spill = A.spill + r * kSpillDepth;
 
   while (logT < (double)kLogCutoff && iter < A.max_iters) {
-    const float cur_t = fabsf(t_state);
-    float bt[K];
-    uint32_t bi[K];
+    uint64_t bk[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      bt[k] = kEmptyT;
-      bi[k] = 0u;
-    }
-    // ---- traversal: gather the K nearest events beyond cur_t ----
+    for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
+    // ---- traversal: gather the K nearest events after the cursor ----
     int sp = 0;
     int32_t node = root;
     int64_t steps = 0;
@@ -169,7 +203,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       if (node >= 0) {
         const float4* nd = A.nodes + 4 * (int64_t)node;
         const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
-        const float t_hi = fminf(A.tmax, bt[K - 1]);
+        const float t_hi = fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
         float tn0, tn1;
         const bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, cur_t, t_hi, tn0);
         const bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, cur_t, t_hi, tn1);
@@ -202,15 +236,17 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
           const float qq = bp + sgnf(bp) * sqrtf(h);
           const float r0 = cc / qq, r1 = qq / aa;
           const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
-          // intersection program (shaders.slang:96-106) + any-hit insertion of entry and exit (:138-159)
+          // intersection program (shaders.slang:96-106) + any-hit insertion of entry and exit (:138-159), with
+          // the (t, key) cursor in place of t > t_cur
           if (!(t1 < cur_t)) {
-            if (cur_t < t0) {
+            const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
+            if (t0 >= cur_t && ve > cursor) {
               if (t0 <= A.tmax) {
-                kbuf_insert<K>(bt, bi, t0, 2u * s + 1u);
-                kbuf_insert<K>(bt, bi, t1, 2u * s);
+                kbuf_insert<K>(bk, ve);
+                kbuf_insert<K>(bk, vx);
               }
-            } else if (cur_t < t1 && t1 <= A.tmax) {
-              kbuf_insert<K>(bt, bi, t1, 2u * s);
+            } else if (vx > cursor && t1 <= A.tmax) {
+              kbuf_insert<K>(bk, vx);
             }
           }
         }
@@ -220,45 +256,53 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
     }
     // ---- integrate the gathered events in order (shaders.slang:210-226) ----
-    bool end = false, stop = false;
-#pragma unroll
+    // The head of the buffer is consumed and the rest shifted down, so the loop body exists once.
+    bool end = false;
     for (int k = 0; k < K; ++k) {
-      if (!stop && bt[k] > kEndT) {
+      const uint64_t head = bk[0];
+#pragma unroll
+      for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
+      const float tk = __uint_as_float((uint32_t)(head >> 32));
+      if (tk > kEndT) {
         end = true;
-        stop = true;
+        break;
       }
-      if (!stop) {
-        const uint32_t s = bi[k] >> 1, face = bi[k] & 1u;
-        const float4 Dr = A.rec[4 * (int64_t)s + 3];
-        const float hgt = face ? Dr.x : -Dr.x;
-        const float t = bt[k];
-        // update (spline-machine.slang:103-138)
-        const float dt = fmaxf(t - t_state, 0.f);
-        const float Df = (float)D0;
-        const float area = fmaxf(Df * dt, 0.f);
-        const float rn0 = safe_div((float)D1, Df), rn1 = safe_div((float)D2, Df), rn2 = safe_div((float)D3, Df);
-        const float e2 = expf(-(float)logT);
-        const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
-        logT = fmax((double)area + logT, 0.0);
-        C0 += w * rn0;
-        C1 += w * rn1;
-        C2 += w * rn2;
-        const float m = ladder((t + t_state) * (0.5f * kPreMulti));
-        dpx += 2.f * w * m * csx;
-        dpy += 2.f * w * csy;
-        csx += w;
-        csy += w * m;
-        D0 += (double)hgt;
-        D1 += (double)(hgt * Dr.y);
-        D2 += (double)(hgt * Dr.z);
-        D3 += (double)(hgt * Dr.w);
-        t_state = t;
-        const uint32_t orig = A.perm[s];
-        if (A.tri) A.tri[r + (int64_t)iter * A.nr] = (int32_t)(2u * orig + face);
-        if (A.touch) atomicAdd(A.touch + orig, 1);
-        ++iter;
-        if (!(logT < (double)kLogCutoff && iter < A.max_iters)) stop = true;
-      }
+      const uint32_t key = (uint32_t)head;
+      const uint32_t s = key >> 1;
+      const bool entry = (key & 1u) == 0u;
+      const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
+      const float4 Dr = A.rec[4 * (int64_t)s + 3];
+      const uint32_t orig = A.perm[s];
+      const f3 mu = ld3(A.means + 3 * (int64_t)orig);
+      const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
+      const float hgt = entry ? Dr.x : -Dr.x;
+      // update (spline-machine.slang:103-138)
+      const float dt = fmaxf((float)(t - t_state), 0.f);
+      const float Df = (float)D0;
+      const float area = fmaxf(Df * dt, 0.f);
+      const float rn0 = safe_div((float)D1, Df), rn1 = safe_div((float)D2, Df), rn2 = safe_div((float)D3, Df);
+      const float e2 = expf(-(float)logT);
+      const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
+      logT = fmax((double)area + logT, 0.0);
+      C0 += w * rn0;
+      C1 += w * rn1;
+      C2 += w * rn2;
+      const float m = ladder((float)((t + t_state) * (0.5 * kPreMulti)));
+      dpx += 2.f * w * m * csx;
+      dpy += 2.f * w * csy;
+      csx += w;
+      csy += w * m;
+      D0 += (double)hgt;
+      D1 += (double)(hgt * Dr.y);
+      D2 += (double)(hgt * Dr.z);
+      D3 += (double)(hgt * Dr.w);
+      t_state = t;
+      cur_t = tk;
+      cursor = head;
+      if (A.tri) A.tri[r + (int64_t)iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
+      if (A.touch) atomicAdd(A.touch + orig, 1);
+      ++iter;
+      if (!(logT < (double)kLogCutoff && iter < A.max_iters)) break;
     }
     if (end) break;
   }
@@ -272,7 +316,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   oc[4] = dpx - dpy;
   float4* os = reinterpret_cast<float4*>(A.states + 16 * r);
   os[0] = make_float4(dpx, dpy, csx, csy);
-  os[1] = make_float4(0.f, 0.f, 0.f, t_state);
+  os[1] = make_float4(0.f, 0.f, 0.f, (float)t_state);
   os[2] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
   os[3] = make_float4((float)logT, C0, C1, C2);
   A.iters[r] = iter;
@@ -282,7 +326,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   s64[2] = D2;
   s64[3] = D3;
   s64[4] = logT;
-  s64[5] = 0.0;
+  s64[5] = t_state;
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,

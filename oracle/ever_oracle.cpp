@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <climits>
 #include <vector>
 #include <omp.h>
 
@@ -446,11 +447,6 @@ template <class R> bool inside_bwd(const Scene<R>& sc, int64_t i, V3<R> p) {
   return std::sqrt(dot(tr, tr)) <= R(1);
 }
 
-struct Event {
-  double t;
-  int id;  // 2*prim + 1 entry, 2*prim exit
-};
-
 template <class R> struct Tracer {
   Scene<R> sc;
   const Bvh* bvh;  // nullable: brute force
@@ -460,9 +456,14 @@ template <class R> struct Tracer {
   bool safe;
 
   // Candidate enumeration restating the any-hit k-buffer (shaders.slang:84-164): collect the kbuf smallest
-  // (t, id) events with t > cur_t, in ascending order (entry before exit of the same primitive on ties).
-  int query(V3<R> o, V3<R> d, R cur_t, std::vector<std::pair<R, int>>& buf, std::vector<int>& stack) const {
+  // events after the cursor, ordered by (t, key) with key = 2*prim + (exit ? 1 : 0) (entry before exit of the
+  // same primitive on equal t). The reference re-queries with a strict t > t_cur (shaders.slang:146), which
+  // drops an event whose t equals the last processed one when the two straddle a k-buffer boundary; the
+  // (t, key) cursor processes tied events exactly once and agrees with the reference everywhere else.
+  int query(V3<R> o, V3<R> d, R cur_t, int64_t cur_key, std::vector<std::pair<R, int64_t>>& buf,
+            std::vector<int>& stack) const {
     buf.clear();
+    auto valid = [&](R t, int64_t key) { return t > cur_t || (t == cur_t && key > cur_key); };
     auto consider = [&](int64_t i) {
       V3<R> ro = o - load3(sc.means + 3 * i);
       const R* q = sc.quats + 4 * i;
@@ -471,13 +472,14 @@ template <class R> struct Tracer {
       R t0, t1;
       eli_roots(Tro, Trd, load3(sc.scales + 3 * i), safe, t0, t1);
       if (t1 < cur_t) return;
-      if (cur_t < t0) {
+      const int64_t ke = 2 * i, kx = 2 * i + 1;
+      if (valid(t0, ke)) {
         if (t0 > tmax) return;  // optixReportHit outside [tmin, tmax] is rejected
-        buf.push_back({t0, (int)(2 * i + 1)});
-        buf.push_back({t1, (int)(2 * i)});
-      } else if (cur_t < t1) {
+        buf.push_back({t0, ke});
+        buf.push_back({t1, kx});
+      } else if (valid(t1, kx)) {
         if (t1 > tmax) return;
-        buf.push_back({t1, (int)(2 * i)});
+        buf.push_back({t1, kx});
       }
     };
     if (!bvh) {
@@ -511,18 +513,14 @@ template <class R> struct Tracer {
       }
     }
     // Exit events may lie beyond tmax when their entry was accepted in the same query (shaders.slang:138-158).
-    std::stable_sort(buf.begin(), buf.end(), [](const std::pair<R, int>& a, const std::pair<R, int>& b) {
-      if (a.first != b.first) return a.first < b.first;
-      if ((a.second >> 1) != (b.second >> 1)) return (a.second >> 1) < (b.second >> 1);
-      return (a.second & 1) > (b.second & 1);  // entry before exit
-    });
+    std::sort(buf.begin(), buf.end());
     if ((int)buf.size() > kbuf) buf.resize(kbuf);
     return (int)buf.size();
   }
 
   R dc_color(int64_t prim, int c) const { return R(kShC0) * sc.feat[prim * sc.d * 3 + c] + R(0.5); }
 
-  // raygen loop (shaders.slang:171-235). Returns the processed event ids and per-event t.
+  // raygen loop (shaders.slang:171-235). Returns the processed event ids (2*prim + entry) and per-event t.
   void trace(V3<R> rayo, V3<R> rayd, const R* init_drgb, State<R>& st, std::vector<int>& ids,
              std::vector<R>* ts) const {
     V3<R> d = l2n(rayd);
@@ -531,25 +529,29 @@ template <class R> struct Tracer {
     for (int i = 0; i < 4; ++i) st.drgb[i] = init_drgb[i];
     ids.clear();
     if (ts) ts->clear();
-    std::vector<std::pair<R, int>> buf;
+    std::vector<std::pair<R, int64_t>> buf;
     std::vector<int> stack;
     int iter = 0;
+    R cur_t = 0;
+    int64_t cur_key = INT64_MAX;  // nothing at t == 0 is an event (t > t_cur initially)
     while (st.logT < R(kLogCutoff) && iter < max_iters) {
-      R start_t = std::fabs(st.t);
-      query(o, d, start_t, buf, stack);
+      query(o, d, cur_t, cur_key, buf, stack);
       bool end = false;
       for (int k = 0; k < kbuf; ++k) {
         if (k >= (int)buf.size() || buf[k].first > R(1e9)) {
           end = true;
           break;
         }
-        int tri = buf[k].second;
-        int64_t prim = tri / 2;
-        R h = (tri % 2 == 1) ? sc.dens[prim] : -sc.dens[prim];
+        const int64_t key = buf[k].second;
+        const int64_t prim = key / 2;
+        const bool entry = (key & 1) == 0;
+        R h = entry ? sc.dens[prim] : -sc.dens[prim];
         R dirac[4] = {h, h * dc_color(prim, 0), h * dc_color(prim, 1), h * dc_color(prim, 2)};
         st = update(st, buf[k].first, dirac);
-        ids.push_back(tri);
+        ids.push_back((int)(2 * prim + (entry ? 1 : 0)));
         if (ts) ts->push_back(buf[k].first);
+        cur_t = buf[k].first;
+        cur_key = key;
         ++iter;
         if (!(st.logT < R(kLogCutoff) && iter < max_iters)) break;
       }

@@ -193,6 +193,18 @@ def test_kbuffer_size_independent():
         np.testing.assert_array_equal(a["iters"], b["iters"])
 
 
+def test_tied_events_across_kbuffer_boundary_processed_once():
+    # two coincident spheres give bit-identical event depths; with a 1-slot k-buffer every event is a chunk
+    # boundary, and the (t, key) cursor must still process all four events
+    scene = (np.array([[1e-7, 0, 1.0], [1e-7, 0, 1.0]]), np.full((2, 3), 0.3), np.tile([1.0, 0, 0, 0], (2, 1)),
+             np.array([1.0, 2.0]), np.array([[[0.1, 0.2, 0.3]], [[-0.3, 0.0, 0.4]]]))
+    a = fwd(scene, *one_ray(), kbuf=1)
+    b = fwd(scene, *one_ray(), kbuf=16)
+    assert a["iters"][0] == b["iters"][0] == 4
+    np.testing.assert_allclose(a["color"], b["color"], rtol=0, atol=0)
+    np.testing.assert_allclose(a["color"][0, 3], 1 - math.exp(-3.0 * 0.6), rtol=1e-12)
+
+
 def test_bvh_matches_brute_force():
     rng = np.random.default_rng(6)
     n = 400
