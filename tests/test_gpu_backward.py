@@ -1,0 +1,132 @@
+"""Backward parity of the HIP adjoint with the FP64 oracle (itself pinned by central differences in
+test_oracle.py). Bar: per-tensor relative L2 error <= 1e-4 and max error <= 1e-3 of the tensor's max."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from ever_amd import scenes
+from ever_amd import tracer as T
+from gpu_util import KEYS, gpu_forward, oracle_forward
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GRADS = ("dmeans", "dscales", "dquats", "ddensities", "dfeatures", "drayo", "drayd")
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def assert_grad_close(a, r, name, l2=1e-4, mx=1e-3):
+    a = np.asarray(a, np.float64)
+    r = np.asarray(r, np.float64)
+    scale = max(np.abs(r).max(), 1e-12)
+    e_l2 = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-12)
+    e_mx = np.abs(a - r).max() / scale
+    assert e_l2 <= l2 and e_mx <= mx, f"{name}: L2 rel {e_l2:.2e}, max rel {e_mx:.2e}"
+
+
+def run_backward(sc, o, d, g, tmin=0.0, max_iters=500, wcts=None, **kw):
+    D = torch.device("cuda:0")
+    scene, out = gpu_forward(sc, o, d, tmin=tmin, max_iters=max_iters, **kw)
+    b = T.trace_backward(scene, o.to(D), d.to(D), tmin, 100.0, max_iters, out, torch.as_tensor(g).float().to(D),
+                         wcts=None if wcts is None else torch.as_tensor(wcts).float().to(D))
+    torch.cuda.synchronize()
+    return out, b
+
+
+def test_config1_gradients_match_golden_oracle():
+    print("case: config1 backward vs golden", flush=True)
+    z = np.load(os.path.join(GOLD, "config1_fp64.npz"))
+    sc, o, d = scenes.config1()
+    _, b = run_backward(sc, o, d, z["grad_out"])
+    for k in GRADS:
+        assert_grad_close(_np(b[k]), z[k], k)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_gradcheck_scenarios_match_golden(case):
+    # tests/grad_check.py:31-99: two rays along +z, one primitive with quaternion (0,0,0,1), tmin 0.3
+    z = np.load(os.path.join(GOLD, "gradcheck_fp64.npz"))
+    sc = {k: torch.tensor(z[f"{case}_{k}"], dtype=torch.float32) for k in KEYS}
+    o = torch.tensor(z[f"{case}_rayo"], dtype=torch.float32)
+    d = torch.tensor(z[f"{case}_rayd"], dtype=torch.float32)
+    out, b = run_backward(sc, o, d, z[f"{case}_grad_out"], tmin=0.3)
+    np.testing.assert_allclose(_np(out["color"])[:, :4], z[f"{case}_color"][:, :4], atol=1e-5)
+    for k in GRADS:
+        assert_grad_close(_np(b[k]), z[f"{case}_{k}"], k)
+
+
+def test_start_inside_gradients():
+    # camera inside the scene, tmin > 0: the start-inside adjoint (backwards_kernel.slang:374-428)
+    sc = scenes.uniform_scene(300, 0, seed=4)
+    rng = np.random.default_rng(4)
+    o = torch.tensor(np.tile(sc["means"][0].numpy(), (256, 1)), dtype=torch.float32)
+    d = torch.tensor(rng.normal(size=(256, 3)), dtype=torch.float32)
+    g = rng.normal(size=(256, 5))
+    out, b = run_backward(sc, o, d, g, tmin=0.01)
+    ref = oracle_forward(sc, o, d, tmin=0.01)
+    assert len(ref["initial_inds"]) > 0 and (ref["initial_drgb"][:, 0] > 0).any()
+    rb = oracle.backward(*[sc[k].double().numpy() for k in KEYS], o.double().numpy(), d.double().numpy(), 0.01,
+                         500, g, ref)
+    for k in GRADS:
+        assert_grad_close(_np(b[k]), rb[k], k)
+
+
+def test_skip_rule_and_clamps():
+    # rays with iters >= max_iters-1 get no gradient (backwards_kernel.slang:304); clamps as the reference
+    sc, o, d = scenes.config1()
+    g = scenes.upstream_grad(o.shape[0]).numpy() * 1e4
+    out, b = run_backward(sc, o, d, g, max_iters=4)
+    ref = oracle_forward(sc, o, d, max_iters=4)
+    rb = oracle.backward(*[sc[k].double().numpy() for k in KEYS], o.double().numpy(), d.double().numpy(), 0.0, 4,
+                         g, ref)
+    it = _np(out["iters"])
+    skip = it >= 3
+    assert skip.any() and (~skip & (it > 0)).any()
+    assert np.all(_np(b["drayo"])[skip] == 0)
+    assert np.abs(_np(b["ddensities"])).max() <= 50 and np.abs(_np(b["dmeans"])).max() <= 1e3
+    for k in GRADS:
+        assert_grad_close(_np(b[k]), rb[k], k)
+
+
+@pytest.mark.parametrize("per_ray", [False, True])
+def test_means2d_statistic(per_ray):
+    # dL/dmeans2D heuristic through inv_project (backwards_kernel.slang:213-220, 244-257)
+    sc, o, d = scenes.config1()
+    W = scenes.world_to_camera((0.0, 0.0, -3.0)).numpy()
+    K = np.eye(4)
+    K[0, 0] = K[1, 1] = 55.4
+    full = (W @ K).astype(np.float64)
+    wcts = np.tile(full[None], (o.shape[0], 1, 1)) if per_ray else full[None]
+    if per_ray:
+        wcts = wcts + np.random.default_rng(0).normal(0, 1e-3, wcts.shape)
+    g = scenes.upstream_grad(o.shape[0]).numpy()
+    out, b = run_backward(sc, o, d, g, wcts=wcts)
+    ref = oracle_forward(sc, o, d)
+    rb = oracle.backward(*[sc[k].double().numpy() for k in KEYS], o.double().numpy(), d.double().numpy(), 0.0, 500,
+                         g, ref, wcts=wcts.astype(np.float32).astype(np.float64))
+    assert_grad_close(_np(b["dmeans2D"]), rb["dmeans2D"], "dmeans2D", l2=1e-3, mx=1e-3)
+
+
+def test_clustered_subset_gradients():
+    # upstream gradient on a ray subset of a 100k clustered scene; DC colours so the oracle chain is direct
+    print("case: clustered 100k backward", flush=True)
+    sc = scenes.clustered_scene(100_000, 0, seed=6)
+    sc["features"] = sc["features"][:, :1].contiguous()
+    o, d = scenes.pinhole_rays(320, 180, (0.0, 0.0, -4.0))
+    nr = o.shape[0]
+    idx = np.arange(0, nr, 37)
+    g = np.zeros((nr, 5))
+    g[idx] = np.random.default_rng(3).normal(size=(len(idx), 5))
+    out, b = run_backward(sc, o, d, g)
+    args = [sc[k].double().numpy() for k in KEYS]
+    ref = oracle.forward(*args, o.double().numpy()[idx], d.double().numpy()[idx], 0.0, 100.0, use_bvh=True)
+    rb = oracle.backward(*args, o.double().numpy()[idx], d.double().numpy()[idx], 0.0, 500, g[idx], ref)
+    for k in ("dmeans", "dscales", "dquats", "ddensities", "dfeatures"):
+        assert_grad_close(_np(b[k]), rb[k], k)
+    for k in ("drayo", "drayd"):
+        assert_grad_close(_np(b[k])[idx], rb[k], k)

@@ -1,0 +1,58 @@
+"""SH colour: the standalone EvalSH mirror (eval_sh.py) and the fused prepass of trace_rays_sh, against the
+FP64 oracle and against the reference's two-step composition eval_sh -> softplus -> RGB2SH -> trace_rays."""
+import numpy as np
+import pytest
+import torch
+
+from ever_amd import eval_sh as E
+from ever_amd import scenes
+from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+D = torch.device("cuda:0")
+C0 = 0.28209479177387814
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_eval_sh_forward_and_vjp(deg):
+    g = torch.Generator().manual_seed(deg)
+    n = 2000
+    means = torch.randn((n, 3), generator=g)
+    feats = torch.randn((n, (deg + 1) ** 2, 3), generator=g)
+    origin = torch.tensor([0.1, -0.2, -2.0])
+    ref = oracle.eval_sh(means.double().numpy(), feats.double().numpy(), deg, origin.double().numpy())
+    f = feats.to(D).requires_grad_(True)
+    out = E.eval_sh(means.to(D), f, origin.to(D), deg)
+    np.testing.assert_allclose(out.detach().cpu().numpy(), ref, atol=2e-6, rtol=1e-5)
+    up = torch.randn((n, 3), generator=g)
+    out.backward(up.to(D))
+    # vjp: d/df_k = Y_k(dir) * up, Y from the oracle's basis (unit features)
+    dirs = means.double().numpy() - origin.double().numpy()
+    for k in range((deg + 1) ** 2):
+        e = np.zeros((n, (deg + 1) ** 2, 3))
+        e[:, k, :] = 1.0
+        yk = oracle.eval_sh(means.double().numpy(), e, deg, origin.double().numpy())[:, 0] - 0.5
+        np.testing.assert_allclose(f.grad[:, k].cpu().numpy(), yk[:, None] * up.numpy(), atol=2e-6, rtol=1e-5)
+    del dirs
+
+
+def test_fused_sh_trace_equals_composition():
+    sc = scenes.uniform_scene(3000, 3, seed=7)
+    o, d = scenes.pinhole_rays(48, 48, (0.3, -0.2, -3.0), target=(0.0, 0.0, 0.0))
+    eye = o[0]
+    ps = [sc[k].to(D).requires_grad_(True) for k in ("means", "scales", "quats", "densities", "features")]
+    g = scenes.upstream_grad(o.shape[0], seed=9).to(D)
+    out = fes.trace_rays_sh(*ps, o.to(D), d.to(D), tuple(eye.tolist()), 3, 0.0, 100.0)
+    (out * g).sum().backward()
+    grads = [p.grad.clone() for p in ps]
+    qs = [sc[k].to(D).requires_grad_(True) for k in ("means", "scales", "quats", "densities", "features")]
+    col = E.eval_sh(qs[0], qs[4], eye.to(D), 3)
+    col = torch.nn.functional.softplus(col, beta=10)
+    f0 = ((col - 0.5) / C0).reshape(-1, 1, 3)
+    out2 = fes.trace_rays(qs[0], qs[1], qs[2], qs[3], f0, o.to(D), d.to(D), 0.0, 100.0)
+    (out2 * g).sum().backward()
+    np.testing.assert_allclose(out.detach().cpu().numpy(), out2.detach().cpu().numpy(), atol=2e-5, rtol=1e-4)
+    for a, b, name in zip(grads, [q.grad for q in qs], ("means", "scales", "quats", "densities", "features")):
+        a, b = a.cpu().numpy().astype(np.float64), b.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b) + 1e-6, name
