@@ -9,7 +9,7 @@ import os
 import torch  # noqa: F401  (loads PyTorch's HIP runtime first, so the library binds to the same runtime)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libever_amd.so")
+LIB_PATH = os.environ.get("EVER_AMD_LIB", os.path.join(_HERE, "libever_amd.so"))
 
 EVER_OK = 0
 EVER_COLOR_DC = 0
@@ -33,12 +33,13 @@ class Rays(ctypes.Structure):
 
 class ForwardOut(ctypes.Structure):
     _fields_ = [("color", _vp), ("states", _vp), ("iters", _vp), ("tri_collection", _vp), ("touch_count", _vp),
-                ("initial_drgb", _vp), ("ray_state64", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t)]
+                ("initial_drgb", _vp), ("ray_state64", _vp), ("ray_order", _vp), ("workspace", _vp),
+                ("workspace_bytes", ctypes.c_size_t)]
 
 
 class BackwardIO(ctypes.Structure):
     _fields_ = [("grad_out", _vp), ("iters", _vp), ("tri_collection", _vp), ("states", _vp), ("ray_state64", _vp),
-                ("wcts", _vp), ("num_wcts", _i64), ("dmeans", _vp), ("dscales", _vp), ("dquats", _vp),
+                ("ray_order", _vp), ("wcts", _vp), ("num_wcts", _i64), ("dmeans", _vp), ("dscales", _vp), ("dquats", _vp),
                 ("ddensities", _vp), ("dfeatures", _vp), ("drayo", _vp), ("drayd", _vp), ("dmeans2D", _vp),
                 ("dinitial_drgb", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t)]
 
@@ -102,12 +103,12 @@ def stream_ptr(device=None):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
-LAYOUT_NAMES = ("header", "aabb", "keys0", "vals0", "keys1", "vals1", "hist", "nodes", "parent", "leaf_parent",
-                "flags", "rec", "colr", "cand", "total")
+LAYOUT_NAMES = ("header", "aabb", "keys0", "vals0", "keys1", "vals1", "hist", "nodes", "range", "sbox", "pre",
+                "suf", "tab", "rec", "colr", "cand", "bpart", "total")
 
 
 def scene_layout(n):
     """Byte offsets of the scene workspace regions (ever_scene_layout)."""
-    o = (_i64 * 15)()
+    o = (_i64 * len(LAYOUT_NAMES))()
     check(lib().ever_scene_layout(n, o), "ever_scene_layout")
     return dict(zip(LAYOUT_NAMES, list(o)))

@@ -14,7 +14,10 @@
 
 namespace ever {
 
-constexpr int kBwdBlock = 64;
+constexpr int kBwdBlock = 256;
+constexpr int kHashSlots = 1024;  // per-block LDS table of primitives touched by the block's rays
+constexpr int kAccFields = 14;    // mean 3, scale 3, quat 4, density 1, colour 3
+constexpr uint32_t kEmptySlot = 0xffffffffu;
 
 struct BwdArgs {
   const float* means;
@@ -38,7 +41,33 @@ struct BwdArgs {
   float* drayo;
   float* drayd;
   float* dinit;      // (R,4) scratch or caller buffer
+  const int32_t* order;  // processing order (nullable: identity)
 };
+
+// Accumulate one event's per-primitive gradient into the block's LDS table (open addressing, linear probing);
+// a primitive that finds no slot within 16 probes goes straight to the global accumulator. The table is
+// flushed to the global accumulator with one atomic per field and primitive at the end of the block, which
+// turns the per-event atomics (contended: neighbouring rays hit the same primitives) into per-block ones.
+__device__ __forceinline__ void scatter_prim(uint32_t* hkeys, float (*hvals)[kHashSlots], float* acc, uint32_t p,
+                                             const float (&v)[kAccFields]) {
+  uint32_t h = (p * 2654435761u) >> 22;  // 10 bits
+  for (int probe = 0; probe < 16; ++probe) {
+    uint32_t k = __hip_atomic_load(hkeys + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == kEmptySlot) {
+      const uint32_t old = atomicCAS(hkeys + h, kEmptySlot, p);
+      k = (old == kEmptySlot) ? p : old;
+    }
+    if (k == p) {
+#pragma unroll
+      for (int j = 0; j < kAccFields; ++j) atomicAdd(&hvals[j][h], v[j]);
+      return;
+    }
+    h = (h + 1) & (kHashSlots - 1);
+  }
+  float* ac = acc + 16 * (int64_t)p;
+#pragma unroll
+  for (int j = 0; j < kAccFields; ++j) atomic_addf(ac + j, v[j]);
+}
 
 __global__ void k_zero_f32(float* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,9 +112,8 @@ __device__ __forceinline__ double event_t(const BwdArgs& A, d3 org, f3 dir, int3
   return safe_root_select(S, id & 1);
 }
 
-__global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
-  const int64_t r = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
-  if (r >= A.nr) return;
+__device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32_t* hkeys,
+                                             float (*hvals)[kHashSlots]) {
   const int E = A.iters[r];
   float4* dinit = reinterpret_cast<float4*>(A.dinit) + r;
   if (E >= A.max_iters - 1 || E <= 0) {  // backwards_kernel.slang:304
@@ -203,21 +231,12 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
     if (safe_roots(rod, dir, scl, q, S)) safe_roots_bwd(S, ro, dir, scl, q, face, tb, rob, rdb, sbar, qb);
     orgb = orgb + rob;
     dirb = dirb + rdb;
+    {
+      const float v[kAccFields] = {-rob.x, -rob.y, -rob.z, sbar.x, sbar.y, sbar.z, qb.x, qb.y, qb.z, qb.w,
+                                   densb, colb0, colb1, colb2};
+      scatter_prim(hkeys, hvals, A.acc, (uint32_t)p, v);
+    }
     float* ac = A.acc + 16 * p;
-    atomic_addf(ac + 0, -rob.x);
-    atomic_addf(ac + 1, -rob.y);
-    atomic_addf(ac + 2, -rob.z);
-    atomic_addf(ac + 3, sbar.x);
-    atomic_addf(ac + 4, sbar.y);
-    atomic_addf(ac + 5, sbar.z);
-    atomic_addf(ac + 6, qb.x);
-    atomic_addf(ac + 7, qb.y);
-    atomic_addf(ac + 8, qb.z);
-    atomic_addf(ac + 9, qb.w);
-    atomic_addf(ac + 10, densb);
-    atomic_addf(ac + 11, colb0);
-    atomic_addf(ac + 12, colb1);
-    atomic_addf(ac + 13, colb2);
     if (inv) {
       // means2D statistic (backwards_kernel.slang:213-220) with this ray's matrix
       const float dist = mean.x * W[2] + mean.y * W[6] + mean.z * W[10] + W[14];
@@ -246,6 +265,30 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   dd[0] = clipf(dirb.x, -1e3f, 1e3f);
   dd[1] = clipf(dirb.y, -1e3f, 1e3f);
   dd[2] = clipf(dirb.z, -1e3f, 1e3f);
+}
+
+__global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
+  __shared__ uint32_t hkeys[kHashSlots];
+  __shared__ float hvals[kAccFields][kHashSlots];
+  for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
+    hkeys[i] = kEmptySlot;
+#pragma unroll
+    for (int j = 0; j < kAccFields; ++j) hvals[j][i] = 0.f;
+  }
+  __syncthreads();
+  const int64_t slot = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
+  if (slot < A.nr) backward_ray(A, A.order ? (int64_t)A.order[slot] : slot, hkeys, hvals);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
+    const uint32_t p = hkeys[i];
+    if (p == kEmptySlot) continue;
+    float* ac = A.acc + 16 * (int64_t)p;
+#pragma unroll
+    for (int j = 0; j < kAccFields; ++j) {
+      const float v = hvals[j][i];
+      if (v != 0.f) atomic_addf(ac + j, v);
+    }
+  }
 }
 
 // backwards_initial_drgb_kernel (backwards_kernel.slang:374-428): ray 0's origin + tmin * raw direction.
@@ -402,6 +445,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     A.drayo = B.drayo;
     A.drayd = B.drayd;
     A.dinit = dinit;
+    A.order = B.ray_order;
     hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
     if (n > 0)
       hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,

@@ -1,6 +1,7 @@
 // LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (AABB, colour, centroid
 // bounds), 30-bit Morton codes, an LDS radix sort written for wave64, Karras' parallel hierarchy
-// construction and an atomic-flag bottom-up refit that also writes the per-primitive traversal records.
+// construction, and node boxes computed as unions over each node's contiguous leaf range (in-block
+// prefix/suffix unions + a sparse table over 256-leaf blocks), which needs no atomics or cross-XCD coherence.
 //
 // Replaces the reference's OptiX acceleration structure (splinetracers/fast_ellipsoid_splinetracer/GAS.cpp:56-153)
 // and its AABB kernel (create_aabbs.cu:69-115), plus the per-primitive SH prepass (splinetracers/slang/sh_kernel.slang:19-42)
@@ -33,7 +34,7 @@ __global__ void k_build_init(uint32_t* header) {
 // Per primitive (original order): AABB of R(q/|q|) diag(s) applied to the unit ball (create_aabbs.cu:77-114),
 // padded by a relative 1e-6 so the box stays conservative under rounding; colour record; centroid bounds.
 __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float* __restrict__ aabb,
-                                                      float4* __restrict__ colr, uint32_t* __restrict__ header) {
+                                                      float4* __restrict__ colr, float* __restrict__ bpart) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = i < P.num_prims;
   float cx = 0.f, cy = 0.f, cz = 0.f;
@@ -76,25 +77,38 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float* __res
     }
     colr[i] = make_float4(dens, c.x, c.y, c.z);
   }
-  // wave-level reduction of centroid bounds, one atomic per wave and component
-  float mnx = valid ? cx : INFINITY, mny = valid ? cy : INFINITY, mnz = valid ? cz : INFINITY;
-  float mxx = valid ? cx : -INFINITY, mxy = valid ? cy : -INFINITY, mxz = valid ? cz : -INFINITY;
-  for (int o = 32; o > 0; o >>= 1) {
-    mnx = fminf(mnx, __shfl_xor(mnx, o));
-    mny = fminf(mny, __shfl_xor(mny, o));
-    mnz = fminf(mnz, __shfl_xor(mnz, o));
-    mxx = fmaxf(mxx, __shfl_xor(mxx, o));
-    mxy = fmaxf(mxy, __shfl_xor(mxy, o));
-    mxz = fmaxf(mxz, __shfl_xor(mxz, o));
+  // block reduction of centroid bounds into a per-block partial (reduced by k_bounds_final)
+  __shared__ float red[6][4];
+  float v[6] = {valid ? cx : INFINITY, valid ? cy : INFINITY, valid ? cz : INFINITY,
+                valid ? -cx : INFINITY, valid ? -cy : INFINITY, valid ? -cz : INFINITY};
+  for (int o = 32; o > 0; o >>= 1)
+    for (int k = 0; k < 6; ++k) v[k] = fminf(v[k], __shfl_xor(v[k], o));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    bpart[6 * (int64_t)blockIdx.x + k] = fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3]));
   }
-  if ((threadIdx.x & 63) == 0 && mnx <= mxx) {
-    atomicMin(header + 0, f2o(mnx));
-    atomicMin(header + 1, f2o(mny));
-    atomicMin(header + 2, f2o(mnz));
-    atomicMax(header + 3, f2o(mxx));
-    atomicMax(header + 4, f2o(mxy));
-    atomicMax(header + 5, f2o(mxz));
+}
+
+// Final reduction of the per-block bounds (one block); writes the ordered-int bounds into the header.
+__global__ __launch_bounds__(256) void k_bounds_final(const float* __restrict__ bpart, int64_t nb,
+                                                      uint32_t* __restrict__ header) {
+  __shared__ float red[6][256];
+  float v[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
+  for (int64_t b = threadIdx.x; b < nb; b += 256)
+    for (int k = 0; k < 6; ++k) v[k] = fminf(v[k], bpart[6 * b + k]);
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = fminf(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+    __syncthreads();
   }
+  if (threadIdx.x < 3) header[threadIdx.x] = f2o(red[threadIdx.x][0]);
+  else if (threadIdx.x < 6) header[threadIdx.x] = f2o(-red[threadIdx.x][0]);
 }
 
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
@@ -230,8 +244,7 @@ __device__ __forceinline__ int kdelta(const uint32_t* keys, int64_t n, int64_t a
 }
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int64_t n,
-                                                float4* __restrict__ nodes, int32_t* __restrict__ parent,
-                                                int32_t* __restrict__ leaf_parent) {
+                                                float4* __restrict__ nodes, int2* __restrict__ range) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -251,43 +264,43 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
   } while (t > 1);
   int64_t gamma = i + s * d + (d < 0 ? d : 0);
   int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-  int32_t left, right;
-  if (lo == gamma) {
-    left = ~(int32_t)gamma;
-    leaf_parent[gamma] = (int32_t)i;
-  } else {
-    left = (int32_t)gamma;
-    parent[gamma] = (int32_t)i;
-  }
-  if (hi == gamma + 1) {
-    right = ~(int32_t)(gamma + 1);
-    leaf_parent[gamma + 1] = (int32_t)i;
-  } else {
-    right = (int32_t)(gamma + 1);
-    parent[gamma + 1] = (int32_t)i;
-  }
+  const int32_t left = (lo == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
+  const int32_t right = (hi == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
   nodes[4 * i + 3] = make_float4(__int_as_float(left), __int_as_float(right), 0.f, 0.f);
-  if (i == 0) parent[0] = -1;
+  range[i] = make_int2((int32_t)lo, (int32_t)hi);
 }
 
-// Per sorted leaf: traversal record, then bottom-up refit. Each internal node holds its two children's boxes;
-// the second child to arrive (atomic counter) merges them and continues to the parent.
+struct Box {
+  float lx, ly, lz, hx, hy, hz;
+};
+__device__ __forceinline__ Box box_empty() { return {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}; }
+__device__ __forceinline__ Box box_union(Box a, Box b) {
+  return {fminf(a.lx, b.lx), fminf(a.ly, b.ly), fminf(a.lz, b.lz),
+          fmaxf(a.hx, b.hx), fmaxf(a.hy, b.hy), fmaxf(a.hz, b.hz)};
+}
+__device__ __forceinline__ Box box_load(const float* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
+__device__ __forceinline__ void box_store(float* p, Box b) {
+  p[0] = b.lx; p[1] = b.ly; p[2] = b.lz; p[3] = b.hx; p[4] = b.hy; p[5] = b.hz;
+}
+
+// Per sorted leaf, blocks of kLeafBlock leaves: the traversal record, the leaf box in sorted order, in-block
+// prefix and suffix unions, and the block union (level 0 of the sparse table).
 //   record (64 B): rows (M_k, b_k), k = 0..2, with M = diag(1/s) L_q, b = -M mean, so that the ellipsoid-frame
 //   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
 //   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
 //   row 3 = (density, R, G, B).
-__global__ __launch_bounds__(256) void k_leaves_refit(ever_prims P, const float* __restrict__ aabb,
-                                                      const float4* __restrict__ colr,
-                                                      const uint32_t* __restrict__ perm,
-                                                      const int32_t* __restrict__ parent,
-                                                      const int32_t* __restrict__ leaf_parent,
-                                                      float4* __restrict__ nodes, uint32_t* __restrict__ flags,
-                                                      float4* __restrict__ rec) {
-  int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t n = P.num_prims;
-  if (sidx >= n) return;
-  uint32_t i = perm[sidx];
-  {
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(ever_prims P, const float* __restrict__ aabb,
+                                                            const float4* __restrict__ colr,
+                                                            const uint32_t* __restrict__ perm,
+                                                            float4* __restrict__ rec, float* __restrict__ sbox,
+                                                            float* __restrict__ pre, float* __restrict__ suf,
+                                                            float* __restrict__ tab0) {
+  __shared__ float sh[6][kLeafBlock];
+  const int64_t n = P.num_prims;
+  const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + threadIdx.x;
+  Box b = box_empty();
+  if (sidx < n) {
+    const uint32_t i = perm[sidx];
     float4 q = reinterpret_cast<const float4*>(P.quats)[i];
     f3 s = ld3(P.scales + 3 * (int64_t)i);
     f3 m = ld3(P.means + 3 * (int64_t)i);
@@ -296,36 +309,116 @@ __global__ __launch_bounds__(256) void k_leaves_refit(ever_prims P, const float*
                      {c0.y / s.y, c1.y / s.y, c2.y / s.y},
                      {c0.z / s.z, c1.z / s.z, c2.z / s.z}};
     for (int k = 0; k < 3; ++k) {
-      float b = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
-      rec[4 * sidx + k] = make_float4(M[k][0], M[k][1], M[k][2], b);
+      float bb = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
+      rec[4 * sidx + k] = make_float4(M[k][0], M[k][1], M[k][2], bb);
     }
     rec[4 * sidx + 3] = colr[i];
+    b = box_load(aabb + 6 * (int64_t)i);
+    box_store(sbox + 6 * sidx, b);
   }
-  if (n < 2) return;
-  const float* b = aabb + 6 * (int64_t)i;
-  float lo0 = b[0], lo1 = b[1], lo2 = b[2], hi0 = b[3], hi1 = b[4], hi2 = b[5];
-  int32_t child = ~(int32_t)sidx;
-  int32_t node = leaf_parent[sidx];
-  while (node >= 0) {
-    float* nf = reinterpret_cast<float*>(nodes + 4 * (int64_t)node);
-    int32_t c0 = __float_as_int(nf[12]);
-    int slot = (c0 == child) ? 0 : 6;
-    nf[slot + 0] = lo0; nf[slot + 1] = lo1; nf[slot + 2] = lo2;
-    nf[slot + 3] = hi0; nf[slot + 4] = hi1; nf[slot + 5] = hi2;
-    uint32_t old = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    int other = 6 - slot;
-    float olo0 = __hip_atomic_load(nf + other + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float olo1 = __hip_atomic_load(nf + other + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float olo2 = __hip_atomic_load(nf + other + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float ohi0 = __hip_atomic_load(nf + other + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float ohi1 = __hip_atomic_load(nf + other + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float ohi2 = __hip_atomic_load(nf + other + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    lo0 = fminf(lo0, olo0); lo1 = fminf(lo1, olo1); lo2 = fminf(lo2, olo2);
-    hi0 = fmaxf(hi0, ohi0); hi1 = fmaxf(hi1, ohi1); hi2 = fmaxf(hi2, ohi2);
-    child = node;
-    node = parent[node];
+  // inclusive prefix (Hillis-Steele) over the block in LDS
+  float v[6] = {b.lx, b.ly, b.lz, -b.hx, -b.hy, -b.hz};  // all as minima
+  for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = 1; o < kLeafBlock; o <<= 1) {
+    float t[6];
+    for (int k = 0; k < 6; ++k) t[k] = threadIdx.x >= o ? sh[k][threadIdx.x - o] : INFINITY;
+    __syncthreads();
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], t[k]);
+    __syncthreads();
   }
+  if (sidx < n) box_store(pre + 6 * sidx, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
+                                           -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
+  if (threadIdx.x == kLeafBlock - 1)
+    box_store(tab0 + 6 * (int64_t)blockIdx.x, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
+                                               -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
+  __syncthreads();
+  // inclusive suffix
+  for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = 1; o < kLeafBlock; o <<= 1) {
+    float t[6];
+    for (int k = 0; k < 6; ++k) t[k] = threadIdx.x + o < kLeafBlock ? sh[k][threadIdx.x + o] : INFINITY;
+    __syncthreads();
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], t[k]);
+    __syncthreads();
+  }
+  if (sidx < n) box_store(suf + 6 * sidx, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
+                                           -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
+}
+
+// Sparse table level k over block unions: tab_k[b] = tab_{k-1}[b] U tab_{k-1}[b + 2^(k-1)].
+__global__ __launch_bounds__(256) void k_table_level(const float* __restrict__ prev, float* __restrict__ cur,
+                                                     int64_t nb, int64_t half) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  Box x = box_load(prev + 6 * b);
+  if (b + half < nb) x = box_union(x, box_load(prev + 6 * (b + half)));
+  box_store(cur + 6 * b, x);
+}
+
+__device__ __forceinline__ Box range_union(int32_t lo, int32_t hi, const float* sbox, const float* pre,
+                                           const float* suf, const float* tab, int64_t nb) {
+  const int32_t bl = lo / kLeafBlock, bh = hi / kLeafBlock;
+  if (bl == bh) {
+    Box x = box_empty();
+    for (int32_t s = lo; s <= hi; ++s) x = box_union(x, box_load(sbox + 6 * (int64_t)s));
+    return x;
+  }
+  Box x = box_union(box_load(suf + 6 * (int64_t)lo), box_load(pre + 6 * (int64_t)hi));
+  const int32_t len = bh - bl - 1;
+  if (len > 0) {
+    const int k = 31 - __clz(len);
+    const float* t = tab + (int64_t)k * 6 * nb;
+    x = box_union(x, box_union(box_load(t + 6 * (int64_t)(bl + 1)), box_load(t + 6 * (int64_t)(bh - (1 << k)))));
+  }
+  return x;
+}
+
+// Per internal node: the boxes of its two children, stored in the node (children's boxes live in the parent).
+__global__ __launch_bounds__(256) void k_node_boxes(float4* __restrict__ nodes, const int2* __restrict__ range,
+                                                    const float* __restrict__ sbox, const float* __restrict__ pre,
+                                                    const float* __restrict__ suf, const float* __restrict__ tab,
+                                                    int64_t n, int64_t nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  float4* nd = nodes + 4 * i;
+  const float4 e = nd[3];
+  Box bx[2];
+  const int32_t ch[2] = {__float_as_int(e.x), __float_as_int(e.y)};
+  for (int c = 0; c < 2; ++c) {
+    if (ch[c] < 0) {
+      bx[c] = box_load(sbox + 6 * (int64_t)(~ch[c]));
+    } else {
+      const int2 r = range[ch[c]];
+      bx[c] = range_union(r.x, r.y, sbox, pre, suf, tab, nb);
+    }
+  }
+  nd[0] = make_float4(bx[0].lx, bx[0].ly, bx[0].lz, bx[0].hx);
+  nd[1] = make_float4(bx[0].hy, bx[0].hz, bx[1].lx, bx[1].ly);
+  nd[2] = make_float4(bx[1].lz, bx[1].hx, bx[1].hy, bx[1].hz);
+}
+
+hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
+                             int bits, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+  int passes = 0;
+  for (int shift = 0; shift < bits; shift += 8, ++passes) {
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, (int64_t)kRadix * tiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
+                       shift, hist, tiles);
+    uint32_t* t = kin; kin = kout; kout = t;
+    t = vin; vin = vout; vout = t;
+  }
+  if (passes & 1) {  // bring the result back to (k0, v0)
+    hipError_t e = hipMemcpyAsync(k0, kin, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(v0, vin, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
@@ -339,33 +432,34 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   uint32_t* v1 = at<uint32_t>(ws, L.vals1);
   uint32_t* hist = at<uint32_t>(ws, L.hist);
   float4* nodes = at<float4>(ws, L.nodes);
-  int32_t* parent = at<int32_t>(ws, L.parent);
-  int32_t* leaf_parent = at<int32_t>(ws, L.leaf_parent);
-  uint32_t* flags = at<uint32_t>(ws, L.flags);
+  int2* range = at<int2>(ws, L.range);
+  float* sbox = at<float>(ws, L.sbox);
+  float* pre = at<float>(ws, L.pre);
+  float* suf = at<float>(ws, L.suf);
+  float* tab = at<float>(ws, L.tab);
   float4* rec = at<float4>(ws, L.rec);
   float4* colr = at<float4>(ws, L.colr);
+  float* bpart = at<float>(ws, L.bpart);
 
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, header);
   if (n <= 0) return hipGetLastError();
-  hipLaunchKernelGGL(k_prim_prepass, dim3(grid_for(n, 256)), dim3(256), 0, st, P, aabb, colr, header);
-  hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, st, aabb, n, header, k0, v0);
-  const int64_t tiles = L.tiles;
-  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
-  for (int shift = 0; shift < 30; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, (int64_t)kRadix * tiles);
-    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
-                       shift, hist, tiles);
-    uint32_t* t = kin; kin = kout; kout = t;
-    t = vin; vin = vout; vout = t;
-  }
-  // four passes: sorted keys/permutation are back in keys0/vals0
-  if (n > 1) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, flags, n - 1, 0u);
-    hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, kin, n, nodes, parent, leaf_parent);
-  }
-  hipLaunchKernelGGL(k_leaves_refit, dim3(grid_for(n, 256)), dim3(256), 0, st, P, aabb, colr, vin, parent,
-                     leaf_parent, nodes, flags, rec);
+  const unsigned nb256 = grid_for(n, 256);
+  hipLaunchKernelGGL(k_prim_prepass, dim3(nb256), dim3(256), 0, st, P, aabb, colr, bpart);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, bpart, (int64_t)nb256, header);
+  hipLaunchKernelGGL(k_morton, dim3(nb256), dim3(256), 0, st, aabb, n, header, k0, v0);
+  hipError_t e = launch_sort_pairs(k0, v0, k1, v1, hist, n, 30, st);
+  if (e != hipSuccess) return e;
+  // sorted codes in keys0, sorted->original permutation in vals0
+  if (n > 1) hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, k0, n, nodes, range);
+  const int64_t nb = L.leaf_blocks;
+  hipLaunchKernelGGL(k_leaf_blocks, dim3((unsigned)nb), dim3(kLeafBlock), 0, st, P, aabb, colr, v0, rec, sbox, pre,
+                     suf, tab);
+  for (int k = 1; k < L.tab_levels; ++k)
+    hipLaunchKernelGGL(k_table_level, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab + (int64_t)(k - 1) * 6 * nb,
+                       tab + (int64_t)k * 6 * nb, nb, (int64_t)1 << (k - 1));
+  if (n > 1)
+    hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, nodes, range, sbox, pre, suf, tab,
+                       n, nb);
   return hipGetLastError();
 }
 

@@ -67,9 +67,9 @@ const char* ever_last_error(void) { return g_last_error.c_str(); }
 int ever_scene_layout(int64_t num_prims, int64_t* o) {
   if (!o) return fail(EVER_ERR_INVALID, "offsets is null");
   ever::SceneLayout L = ever::scene_layout(num_prims > 0 ? num_prims : 0);
-  const size_t v[15] = {L.header, L.aabb, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes,
-                        L.parent, L.leaf_parent, L.flags, L.rec, L.colr, L.cand, L.total};
-  for (int i = 0; i < 15; ++i) o[i] = (int64_t)v[i];
+  const size_t v[18] = {L.header, L.aabb, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes, L.range,
+                        L.sbox, L.pre, L.suf, L.tab, L.rec, L.colr, L.cand, L.bpart, L.total};
+  for (int i = 0; i < 18; ++i) o[i] = (int64_t)v[i];
   return EVER_OK;
 }
 
@@ -77,9 +77,7 @@ size_t ever_scene_workspace_bytes(int64_t num_prims) {
   return ever::scene_layout(num_prims > 0 ? num_prims : 0).total;
 }
 
-size_t ever_forward_workspace_bytes(int64_t num_rays) {
-  return ever::align_up(sizeof(int32_t) * ever::kSpillPerRay * (size_t)(num_rays > 0 ? num_rays : 1), 256);
-}
+size_t ever_forward_workspace_bytes(int64_t num_rays) { return ever::forward_layout(num_rays).total; }
 
 size_t ever_backward_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts) {
   return ever::backward_layout(num_prims, num_rays, num_wcts).total;
@@ -102,13 +100,12 @@ int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspac
   if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
     return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
   if (rays->num_rays > 0) {
-    if (!out->color || !out->states || !out->iters || !out->initial_drgb || !out->ray_state64)
+    if (!out->color || !out->states || !out->iters || !out->initial_drgb || !out->ray_state64 || !out->ray_order)
       return fail(EVER_ERR_INVALID, "a required forward output is null");
     if (!out->workspace || out->workspace_bytes < ever_forward_workspace_bytes(rays->num_rays))
       return fail(EVER_ERR_WORKSPACE, "forward workspace too small");
   }
-  return hip_status(ever::launch_forward(*prims, workspace, *rays, *out, static_cast<int32_t*>(out->workspace),
-                                         (hipStream_t)stream),
+  return hip_status(ever::launch_forward(*prims, workspace, *rays, *out, (hipStream_t)stream),
                     "ever_trace_forward");
 }
 

@@ -46,6 +46,7 @@ struct FwdArgs {
   float* init_drgb;
   double* st64;
   int32_t* spill;
+  const uint32_t* order;  // processing order: thread slot -> ray index
 };
 
 // kern_prefilter (initialize_density.cu:69-105): primitives whose AABB lies within tmin of ray 0's origin.
@@ -68,6 +69,50 @@ __global__ __launch_bounds__(256) void k_prefilter(const float* __restrict__ aab
 }
 
 __global__ void k_reset_count(uint32_t* header) { header[8] = 0u; }
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+// Sort key of a ray: 30-bit Morton code of the point o + d * |c - o| (c = centre of the primitive centroids),
+// quantized over a cube of four times the scene extent around c. Rays of neighbouring pixels of one camera map
+// to neighbouring points, so a wave traces a compact screen tile; rays from different origins separate.
+__global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ rayo, const float* __restrict__ rayd,
+                                                  int64_t nr, float tmin, const uint32_t* __restrict__ header,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  auto o2f = [](uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); };
+  const f3 lo = mk3(o2f(header[0]), o2f(header[1]), o2f(header[2]));
+  const f3 hi = mk3(o2f(header[3]), o2f(header[4]), o2f(header[5]));
+  const f3 c = 0.5f * (lo + hi);
+  const float ext = fmaxf(fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z), 1e-6f);
+  const f3 rd = ld3(rayd + 3 * r);
+  const f3 o = ld3(rayo + 3 * r) + tmin * rd;
+  const f3 d = l2n(rd);
+  const f3 co = c - o;
+  const f3 p = o + sqrtf(dot3(co, co)) * d;
+  const float inv = 1.f / (4.f * ext);
+  const uint32_t qx = (uint32_t)fminf(fmaxf(((p.x - c.x) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+  const uint32_t qy = (uint32_t)fminf(fmaxf(((p.y - c.y) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+  const uint32_t qz = (uint32_t)fminf(fmaxf(((p.z - c.z) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+  keys[r] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+  vals[r] = (uint32_t)r;
+}
+
+__global__ void k_iota(uint32_t* v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ a, int32_t* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (int32_t)a[i];
+}
 
 __global__ void k_zero_i32(int32_t* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -129,8 +174,9 @@ template <int K>
 __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   __shared__ int32_t stk[kLdsDepth * kFwdBlock];
   const int lane = threadIdx.x;
-  const int64_t r = (int64_t)blockIdx.x * kFwdBlock + lane;
-  if (r >= A.nr) return;
+  const int64_t slot = (int64_t)blockIdx.x * kFwdBlock + lane;
+  if (slot >= A.nr) return;
+  const int64_t r = A.order[slot];
 
   const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
   const f3 d = l2n(rdraw);
@@ -188,7 +234,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   uint64_t cursor = pack_event(0.f, 0xffffffffu);
   int iter = 0;
   const int32_t root = A.n == 1 ? ~0 : 0;
-  int32_t* spill = A.spill + r * kSpillDepth;
+  int32_t* spill = A.spill + slot * kSpillDepth;
 
   while (logT < (double)kLogCutoff && iter < A.max_iters) {
     uint64_t bk[K];
@@ -330,7 +376,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
-                          int32_t* spill, hipStream_t st) {
+                          hipStream_t st) {
   SceneLayout L = scene_layout(P.num_prims);
   uint32_t* header = at<uint32_t>(ws, L.header);
   int32_t* cand = at<int32_t>(ws, L.cand);
@@ -341,6 +387,20 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   if (P.num_prims > 0)
     hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, at<float>(ws, L.aabb),
                        P.num_prims, R.origins, R.tmin, cand, header);
+  // processing order
+  ForwardLayout FL = forward_layout(R.num_rays);
+  uint32_t* k0 = at<uint32_t>(O.workspace, FL.keys0);
+  uint32_t* v0 = at<uint32_t>(O.workspace, FL.vals0);
+  if (R.num_rays >= kMinRaysToSort && P.num_prims > 0) {
+    hipLaunchKernelGGL(k_ray_keys, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, R.origins, R.directions,
+                       R.num_rays, R.tmin, header, k0, v0);
+    hipError_t e = launch_sort_pairs(k0, v0, at<uint32_t>(O.workspace, FL.keys1), at<uint32_t>(O.workspace, FL.vals1),
+                                     at<uint32_t>(O.workspace, FL.hist), R.num_rays, 30, st);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, v0, R.num_rays);
+  }
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, v0, O.ray_order, R.num_rays);
   FwdArgs A;
   A.rec = at<float4>(ws, L.rec);
   A.nodes = at<float4>(ws, L.nodes);
@@ -368,7 +428,8 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.touch = O.touch_count;
   A.init_drgb = O.initial_drgb;
   A.st64 = O.ray_state64;
-  A.spill = spill;
+  A.spill = at<int32_t>(O.workspace, FL.spill);
+  A.order = v0;
   dim3 grid(grid_for(R.num_rays, kFwdBlock));
   if (P.num_prims <= 0) {
     // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0

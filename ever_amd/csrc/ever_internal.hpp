@@ -17,25 +17,35 @@ constexpr int kRadix = 256;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+constexpr int kLeafBlock = 256;  // leaves per block of the range-union refit
+
 // Scene workspace: everything the LBVH build writes and the tracers read. All offsets 256-byte aligned.
 //   header   : uint32[16]  (0..5 centroid bounds as ordered ints, 8 = start-inside candidate count)
 //   aabb     : float[N*6]  per-primitive box (original order)
-//   keys0/1, vals0/1 : uint32[N] radix-sort ping-pong (vals end as the sorted->original permutation)
+//   keys0/1, vals0/1 : uint32[N] radix-sort ping-pong (vals0 ends as the sorted->original permutation)
 //   hist     : uint32[256 * tiles]
 //   nodes    : float4[(N-1) * 4] BVH2 nodes, children boxes stored in the parent (64 B per node)
-//   parent   : int32[N-1], leaf_parent : int32[N], flags : uint32[N-1]
+//   range    : int2[N-1] leaf range of each internal node
+//   sbox/pre/suf : float[N*6] leaf boxes in sorted order, in-block prefix and suffix unions
+//   tab      : float[levels * blocks * 6] sparse table of block unions
 //   rec      : float4[N * 4] per-primitive traversal record in LBVH order (64 B)
 //   colr     : float4[N] (density, R, G, B) in original order
 //   cand     : int32[N] start-inside candidates
+//   bpart    : float[6 * ceil(N/256)] per-block centroid bounds
 struct SceneLayout {
-  int64_t n = 0, tiles = 0;
-  size_t header, aabb, keys0, vals0, keys1, vals1, hist, nodes, parent, leaf_parent, flags, rec, colr, cand, total;
+  int64_t n = 0, tiles = 0, leaf_blocks = 0;
+  int tab_levels = 1;
+  size_t header, aabb, keys0, vals0, keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand, bpart,
+      total;
 };
 
 inline SceneLayout scene_layout(int64_t n) {
   SceneLayout L;
   L.n = n;
   L.tiles = (n + kSortTile - 1) / kSortTile;
+  L.leaf_blocks = (n + kLeafBlock - 1) / kLeafBlock;
+  L.tab_levels = 1;
+  while ((int64_t(1) << L.tab_levels) <= L.leaf_blocks) ++L.tab_levels;
   int64_t ni = n > 1 ? n - 1 : 1;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -51,12 +61,15 @@ inline SceneLayout scene_layout(int64_t n) {
   L.vals1 = take(4 * n);
   L.hist = take(4 * (size_t)kRadix * (L.tiles > 0 ? L.tiles : 1));
   L.nodes = take(64 * ni);
-  L.parent = take(4 * ni);
-  L.leaf_parent = take(4 * n);
-  L.flags = take(4 * ni);
+  L.range = take(8 * ni);
+  L.sbox = take(24 * n);
+  L.pre = take(24 * n);
+  L.suf = take(24 * n);
+  L.tab = take(24 * (size_t)L.tab_levels * (L.leaf_blocks > 0 ? L.leaf_blocks : 1));
   L.rec = take(64 * n);
   L.colr = take(16 * n);
   L.cand = take(4 * n);
+  L.bpart = take(24 * (size_t)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1));
   L.total = off;
   return L;
 }
@@ -86,9 +99,12 @@ template <class T> inline const T* at(const void* base, size_t off) {
 }
 
 // launchers (return hipSuccess or the first error)
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits (8 per pass); the result ends in (k0, v0).
+hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
+                             int bits, hipStream_t st);
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t s);
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
-                          int32_t* spill, hipStream_t s);
+                          hipStream_t s);
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
                            hipStream_t s);
 hipError_t launch_eval_sh(const float* means, const float* feat, const float* origin, int64_t n, int d, int deg,
@@ -97,6 +113,32 @@ hipError_t launch_eval_sh_bwd(const float* means, const float* feat, const float
                               int64_t n, int d, int deg, float* dfeat, hipStream_t s);
 
 constexpr int kSpillPerRay = 32;  // int32 traversal-stack spill entries per ray
+constexpr int64_t kMinRaysToSort = 65536;  // below this the rays are traced in the caller's order
+
+// Forward workspace: traversal-stack spill per ray, and the ray sort (30-bit Morton key of a point on each ray).
+struct ForwardLayout {
+  int64_t tiles = 0;
+  size_t spill, keys0, vals0, keys1, vals1, hist, total;
+};
+inline ForwardLayout forward_layout(int64_t nr) {
+  ForwardLayout L;
+  const int64_t r = nr > 0 ? nr : 1;
+  L.tiles = (r + kSortTile - 1) / kSortTile;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  L.spill = take(sizeof(int32_t) * kSpillPerRay * (size_t)r);
+  L.keys0 = take(4 * (size_t)r);
+  L.vals0 = take(4 * (size_t)r);
+  L.keys1 = take(4 * (size_t)r);
+  L.vals1 = take(4 * (size_t)r);
+  L.hist = take(4 * (size_t)kRadix * L.tiles);
+  L.total = off;
+  return L;
+}
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

@@ -44,7 +44,8 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     ctx.tmax = tmax
     ctx.kbuf = T.get_kbuf()
     ctx.save_for_backward(mean, scale, quat, density, color, rayo, rayd, out["tri_collection"], wcts,
-                          out["initial_drgb"], out["states"], out["iters"], out["ray_state64"], scene.workspace)
+                          out["initial_drgb"], out["states"], out["iters"], out["ray_state64"], out["ray_order"],
+                          scene.workspace)
     color_and_loss = out["color"]
     if return_extras:
         nr = rayo.shape[0]
@@ -68,9 +69,9 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
 
 def _run_backward(ctx, grad_output):
     (mean, scale, quat, density, features, rayo, rayd, tri_collection, wcts, initial_drgb, states, iters, st64,
-     _ws) = ctx.saved_tensors
+     order, _ws) = ctx.saved_tensors
     fwd = dict(tri_collection=tri_collection, states=states, iters=iters, ray_state64=st64,
-               initial_drgb=initial_drgb)
+               initial_drgb=initial_drgb, ray_order=order)
     g = T.trace_backward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, fwd, grad_output, wcts=wcts,
                          kbuf=ctx.kbuf)
     return (g["dmeans"], g["dscales"], g["dquats"], g["ddensities"].reshape(density.shape), g["dfeatures"],

@@ -164,18 +164,19 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     touch = torch.empty((scene.n,), dtype=torch.int32, device=dev)
     init = torch.empty((nr, 4), dtype=torch.float32, device=dev)
     st64 = torch.empty((nr, 6), dtype=torch.float64, device=dev)
+    order = torch.empty((nr,), dtype=torch.int32, device=dev)
     fws_bytes = L.ever_forward_workspace_bytes(nr)
     fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
-                       C.ptr(st64), C.ptr(fws), fws_bytes)
+                       C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
         tok = _t0("forward")
         C.check(L.ever_trace_forward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
                                      ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
         _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
-                ray_state64=st64, rays=rays)
+                ray_state64=st64, ray_order=order, rays=rays)
 
 
 def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
@@ -206,7 +207,7 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
     io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
-                      C.ptr(fwd["ray_state64"]), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
+                      C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
                       C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),
                       C.ptr(out["drayd"]), C.ptr(out.get("dmeans2D")), C.ptr(out["dinitial_drgb"]), C.ptr(ws),
                       ws_bytes)

@@ -76,7 +76,8 @@ typedef struct ever_forward_out {
   int32_t* touch_count;    /* (N) events per primitive (exact); nullable; zeroed by the call */
   float* initial_drgb;     /* (R,4) density/colour already open at the ray start */
   double* ray_state64;     /* (R,6) FP64 final accumulators, read by ever_trace_backward */
-  void* workspace;         /* traversal-stack spill, ever_forward_workspace_bytes(R) bytes */
+  int32_t* ray_order;      /* (R) processing order (rays sorted for coherence), read by ever_trace_backward */
+  void* workspace;         /* stack spill + ray sort, ever_forward_workspace_bytes(R) bytes */
   size_t workspace_bytes;
 } ever_forward_out;
 
@@ -86,6 +87,7 @@ typedef struct ever_backward_io {
   const int32_t* tri_collection;
   const float* states;
   const double* ray_state64;
+  const int32_t* ray_order; /* forward's processing order; nullable (identity) */
   const float* wcts; /* (W,4,4) world-to-camera matrices for the means2D statistic; nullable */
   int64_t num_wcts;
   float* dmeans;     /* (N,3) */
@@ -107,8 +109,8 @@ int ever_struct_sizes(int64_t* out4);
 const char* ever_last_error(void);
 
 /* Byte offsets of the scene workspace regions, for inspection by tests (order: header, aabb, keys0, vals0,
- * keys1, vals1, hist, nodes, parent, leaf_parent, flags, rec, colr, cand, total). */
-int ever_scene_layout(int64_t num_prims, int64_t* offsets15);
+ * keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand, bpart, total). */
+int ever_scene_layout(int64_t num_prims, int64_t* offsets18);
 
 /* Scene workspace: LBVH nodes, sort buffers, per-primitive records. Lives from build to backward. */
 size_t ever_scene_workspace_bytes(int64_t num_prims);
