@@ -343,7 +343,36 @@ struct EpiArgs {
   float* ddens;
   float* dfeat;
   float* dmeans2D;
+  float* grgb;  // (N,4) colour-space gradient before the SH basis (colour mode EVER_COLOR_SH)
 };
+
+// dL/df[p,k,c] = Y_k(dir_p) * g[p,c], one lane per output element so the (N, d, 3) writes are coalesced.
+__global__ __launch_bounds__(256) void k_sh_feature_grads(const float* __restrict__ grgb,
+                                                          const float* __restrict__ means, float ox, float oy,
+                                                          float oz, int64_t n, int d, int deg,
+                                                          float* __restrict__ dfeat) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = 3 * (int64_t)d;
+  if (e >= n * per) return;
+  const int64_t p = e / per;
+  const int j = (int)(e - p * per);
+  const int k = j / 3, ch = j - 3 * k;
+  const int nc = (deg + 1) * (deg + 1);
+  float v = 0.f;
+  if (k < nc) {
+    const float g = grgb[4 * p + ch];
+    if (g != 0.f) {
+      const f3 dir = sh_dir(ld3(means + 3 * p), mk3(ox, oy, oz));
+      float bas[16];
+      sh_basis(deg, dir.x, dir.y, dir.z, bas);
+      float bk = bas[0];
+#pragma unroll
+      for (int t = 1; t < 16; ++t) bk = (t == k) ? bas[t] : bk;
+      v = bk * g;
+    }
+  }
+  dfeat[e] = v;
+}
 
 // Per primitive: clamps (fast_ellipsoid_splinetracer.py:201-211), colour -> SH features, means2D.
 __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
@@ -368,14 +397,14 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
               g2 = clipf(kShC0 * a3.y, -1e3f, 1e3f);
   float* df = E.dfeat + (int64_t)3 * E.d * p;
   if (E.color_mode == EVER_COLOR_SH) {
-    // chain through RGB2SH (1/SH_C0), the optional softplus and the SH basis (sh_kernel.slang:45-98)
+    // chain through RGB2SH (1/SH_C0) and the optional softplus; the SH basis is applied by k_sh_feature_grads
     const f3 mean = ld3(E.means + 3 * p);
-    const f3 dir = sh_dir(mean, mk3(E.ox, E.oy, E.oz));
-    float bas[16];
-    sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
-    const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
     float gr[3] = {g0 / kShC0, g1 / kShC0, g2 / kShC0};
     if (E.activation == 1) {
+      const f3 dir = sh_dir(mean, mk3(E.ox, E.oy, E.oz));
+      float bas[16];
+      sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
+      const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
       const float* f = E.feat + (int64_t)3 * E.d * p;
       for (int ch = 0; ch < 3; ++ch) {
         float x = 0.5f;
@@ -383,9 +412,9 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
         gr[ch] *= softplus10_grad(x);
       }
     }
+    // bw_sh_kernel skips primitives whose colour gradient is below 1e-8 in every channel (sh_kernel.slang:167-170)
     const bool skip = fabsf(gr[0]) < 1e-8f && fabsf(gr[1]) < 1e-8f && fabsf(gr[2]) < 1e-8f;
-    for (int k = 0; k < E.d; ++k)
-      for (int ch = 0; ch < 3; ++ch) df[3 * k + ch] = (!skip && k < nc) ? bas[k] * gr[ch] : 0.f;
+    reinterpret_cast<float4*>(E.grgb)[p] = skip ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(gr[0], gr[1], gr[2], 0.f);
   } else {
     df[0] = g0;
     df[1] = g1;
@@ -474,7 +503,12 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     E.ddens = B.ddensities;
     E.dfeat = B.dfeatures;
     E.dmeans2D = nw > 0 ? B.dmeans2D : nullptr;
+    E.grgb = at<float>(B.workspace, BL.grgb);
     hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);
+    if (P.color_mode == EVER_COLOR_SH)
+      hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n * 3 * (int64_t)P.feature_size, 256)), dim3(256), 0, st,
+                         E.grgb, P.means, P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size,
+                         P.sh_degree, B.dfeatures);
   }
   return hipGetLastError();
 }

@@ -1,5 +1,5 @@
-// LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (AABB, colour, centroid
-// bounds), 30-bit Morton codes, an LDS radix sort written for wave64, Karras' parallel hierarchy
+// LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (colour, traversal record,
+// centroid bounds), leaf fragments (elongated ellipsoids split along their major axis), 30-bit Morton codes, an LDS radix sort written for wave64, Karras' parallel hierarchy
 // construction, and node boxes computed as unions over each node's contiguous leaf range (in-block
 // prefix/suffix unions + a sparse table over 256-leaf blocks), which needs no atomics or cross-XCD coherence.
 //
@@ -31,31 +31,20 @@ __global__ void k_build_init(uint32_t* header) {
   else if (t < 16) header[t] = 0u;
 }
 
-// Per primitive (original order): AABB of R(q/|q|) diag(s) applied to the unit ball (create_aabbs.cu:77-114),
-// padded by a relative 1e-6 so the box stays conservative under rounding; colour record; centroid bounds.
-__global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float* __restrict__ aabb,
+// Per primitive (original order): colour record, traversal record and centroid bounds.
+//   record (64 B): rows (M_k, b_k), k = 0..2, with M = diag(1/s) L_q, b = -M mean, so that the ellipsoid-frame
+//   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
+//   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
+//   row 3 = (density, R, G, B).
+__global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __restrict__ rec,
                                                       float4* __restrict__ colr, float* __restrict__ bpart) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = i < P.num_prims;
   float cx = 0.f, cy = 0.f, cz = 0.f;
   if (valid) {
-    float4 q = reinterpret_cast<const float4*>(P.quats)[i];
-    float qn = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    float r = q.x * qn, x = q.y * qn, y = q.z * qn, z = q.w * qn;
-    float R00 = 1.f - 2.f * (y * y + z * z), R01 = 2.f * (x * y - r * z), R02 = 2.f * (x * z + r * y);
-    float R10 = 2.f * (x * y + r * z), R11 = 1.f - 2.f * (x * x + z * z), R12 = 2.f * (y * z - r * x);
-    float R20 = 2.f * (x * z - r * y), R21 = 2.f * (y * z + r * x), R22 = 1.f - 2.f * (x * x + y * y);
-    f3 s = ld3(P.scales + 3 * i);
-    f3 m = ld3(P.means + 3 * i);
-    float ex = sqrtf(R00 * R00 * s.x * s.x + R01 * R01 * s.y * s.y + R02 * R02 * s.z * s.z);
-    float ey = sqrtf(R10 * R10 * s.x * s.x + R11 * R11 * s.y * s.y + R12 * R12 * s.z * s.z);
-    float ez = sqrtf(R20 * R20 * s.x * s.x + R21 * R21 * s.y * s.y + R22 * R22 * s.z * s.z);
-    ex = ex * (1.f + 1e-6f) + 1e-7f * fabsf(m.x);
-    ey = ey * (1.f + 1e-6f) + 1e-7f * fabsf(m.y);
-    ez = ez * (1.f + 1e-6f) + 1e-7f * fabsf(m.z);
-    float* b = aabb + 6 * i;
-    b[0] = m.x - ex; b[1] = m.y - ey; b[2] = m.z - ez;
-    b[3] = m.x + ex; b[4] = m.y + ey; b[5] = m.z + ez;
+    const float4 q = reinterpret_cast<const float4*>(P.quats)[i];
+    const f3 s = ld3(P.scales + 3 * i);
+    const f3 m = ld3(P.means + 3 * i);
     cx = m.x; cy = m.y; cz = m.z;
     float dens = P.densities[i];
     float3 c;
@@ -75,7 +64,17 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float* __res
     } else {
       c = make_float3(kShC0 * f[0] + 0.5f, kShC0 * f[1] + 0.5f, kShC0 * f[2] + 0.5f);
     }
-    colr[i] = make_float4(dens, c.x, c.y, c.z);
+    const float4 cr = make_float4(dens, c.x, c.y, c.z);
+    colr[i] = cr;
+    const f3 c0 = qrot(mk3(1.f, 0.f, 0.f), q), c1 = qrot(mk3(0.f, 1.f, 0.f), q), c2 = qrot(mk3(0.f, 0.f, 1.f), q);
+    const float M[3][3] = {{c0.x / s.x, c1.x / s.x, c2.x / s.x},
+                           {c0.y / s.y, c1.y / s.y, c2.y / s.y},
+                           {c0.z / s.z, c1.z / s.z, c2.z / s.z}};
+    for (int k = 0; k < 3; ++k) {
+      const float bb = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
+      rec[4 * i + k] = make_float4(M[k][0], M[k][1], M[k][2], bb);
+    }
+    rec[4 * i + 3] = cr;
   }
   // block reduction of centroid bounds into a per-block partial (reduced by k_bounds_final)
   __shared__ float red[6][4];
@@ -119,23 +118,83 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ aabb, int64_t n,
-                                                const uint32_t* __restrict__ header, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float lo[3] = {o2f(header[0]), o2f(header[1]), o2f(header[2])};
-  float hi[3] = {o2f(header[3]), o2f(header[4]), o2f(header[5])};
-  const float* b = aabb + 6 * i;
-  uint32_t q[3];
-  for (int k = 0; k < 3; ++k) {
-    float ext = hi[k] - lo[k];
-    float c = 0.5f * (b[k] + b[3 + k]);
-    float u = ext > 0.f ? (c - lo[k]) / ext : 0.5f;
-    q[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+// Leaf fragments of one primitive: the ellipsoid is cut along its longest local axis into
+// F = clamp(ceil(s_max / s_mid), 1, kMaxFragments) slabs; fragment j's box is the intersection of the
+// ellipsoid's tight AABB (create_aabbs.cu:77-114: half-extent_k = sqrt(sum_l (R_kl s_l)^2), q normalized) and
+// the AABB of the local box [a_j, b_j] x [-rho_j, rho_j]^2 that contains the slab (rho_j = sqrt(1 - min a^2)).
+// Boxes are padded by a relative 1e-6 so they stay conservative under rounding. Each fragment gets the 30-bit
+// Morton code of its box centre; unused slots get key 0xffffffff. The leaf count L is accumulated in header[9].
+__global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* __restrict__ header,
+                                                        float* __restrict__ aabb, uint32_t* __restrict__ fprim,
+                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  __shared__ uint32_t cnt[4];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int F = 0;
+  if (i < P.num_prims) {
+    float lo[3] = {o2f(header[0]), o2f(header[1]), o2f(header[2])};
+    float hi[3] = {o2f(header[3]), o2f(header[4]), o2f(header[5])};
+    const float4 q = reinterpret_cast<const float4*>(P.quats)[i];
+    const float qn = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    const float r = q.x * qn, x = q.y * qn, y = q.z * qn, z = q.w * qn;
+    const float Rm[3][3] = {{1.f - 2.f * (y * y + z * z), 2.f * (x * y - r * z), 2.f * (x * z + r * y)},
+                            {2.f * (x * y + r * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z - r * x)},
+                            {2.f * (x * z - r * y), 2.f * (y * z + r * x), 1.f - 2.f * (x * x + y * y)}};
+    const float sv[3] = {P.scales[3 * i], P.scales[3 * i + 1], P.scales[3 * i + 2]};
+    const float mv[3] = {P.means[3 * i], P.means[3 * i + 1], P.means[3 * i + 2]};
+    int ax = 0;
+    if (sv[1] > sv[ax]) ax = 1;
+    if (sv[2] > sv[ax]) ax = 2;
+    float mid = 0.f;
+    for (int k = 0; k < 3; ++k)
+      if (k != ax) mid = fmaxf(mid, sv[k]);
+    F = mid > 0.f ? (int)ceilf(sv[ax] / mid) : kMaxFragments;
+    F = F < 1 ? 1 : (F > kMaxFragments ? kMaxFragments : F);
+    float tight[3];
+    for (int k = 0; k < 3; ++k)
+      tight[k] = sqrtf(Rm[k][0] * Rm[k][0] * sv[0] * sv[0] + Rm[k][1] * Rm[k][1] * sv[1] * sv[1] +
+                       Rm[k][2] * Rm[k][2] * sv[2] * sv[2]);
+    for (int j = 0; j < kMaxFragments; ++j) {
+      const int64_t slot = (int64_t)kMaxFragments * i + j;
+      vals[slot] = (uint32_t)slot;
+      if (j >= F) {
+        keys[slot] = 0xffffffffu;
+        continue;
+      }
+      const float a = -1.f + 2.f * j / F, b = -1.f + 2.f * (j + 1) / F;
+      const float mn = (a <= 0.f && b >= 0.f) ? 0.f : fminf(a * a, b * b);
+      const float rho = sqrtf(fmaxf(1.f - mn, 0.f));
+      float cl[3] = {0.f, 0.f, 0.f}, hl[3] = {rho, rho, rho};
+      cl[ax] = 0.5f * (a + b);
+      hl[ax] = 0.5f * (b - a);
+      float* bx = aabb + 6 * slot;
+      uint32_t qk[3];
+      for (int k = 0; k < 3; ++k) {
+        float cc = mv[k], e = 0.f;
+        for (int l = 0; l < 3; ++l) {
+          cc += Rm[k][l] * sv[l] * cl[l];
+          e += fabsf(Rm[k][l] * sv[l]) * hl[l];
+        }
+        const float lo_k = fmaxf(cc - e, mv[k] - tight[k]), hi_k = fminf(cc + e, mv[k] + tight[k]);
+        const float pad = 1e-6f * fmaxf(hi_k - lo_k, tight[k]) + 1e-7f * fabsf(mv[k]);
+        bx[k] = lo_k - pad;
+        bx[3 + k] = hi_k + pad;
+        const float ext = hi[k] - lo[k];
+        const float u = ext > 0.f ? (0.5f * (lo_k + hi_k) - lo[k]) / ext : 0.5f;
+        qk[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+      }
+      keys[slot] = (expand_bits10(qk[0]) << 2) | (expand_bits10(qk[1]) << 1) | expand_bits10(qk[2]);
+      fprim[slot] = (uint32_t)i;
+    }
   }
-  keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
-  vals[i] = (uint32_t)i;
+  // leaf count: wave sums, one atomic per block
+  int v = F;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = (uint32_t)v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    if (t) atomicAdd(header + 9, t);
+  }
 }
 
 // ---- LSD radix sort, 8-bit digits, tiles of kSortTile keys --------------------------------------------------
@@ -155,28 +214,31 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
   hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of hist (length m) by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ a, int64_t m) {
-  __shared__ uint32_t part[1024];
-  int t = threadIdx.x;
-  int64_t chunk = (m + 1023) / 1024;
-  int64_t b = t * chunk, e = b + chunk < m ? b + chunk : m;
+// Exclusive scan of each digit's row of the histogram (one block per digit), and the digit totals.
+__global__ __launch_bounds__(256) void k_scan_digits(uint32_t* __restrict__ hist, int64_t tiles,
+                                                     uint32_t* __restrict__ totals) {
+  __shared__ uint32_t part[256];
+  const int t = threadIdx.x;
+  uint32_t* row = hist + (int64_t)blockIdx.x * tiles;
+  const int64_t chunk = (tiles + 255) / 256;
+  const int64_t b = t * chunk, e = b + chunk < tiles ? b + chunk : tiles;
   uint32_t s = 0;
-  for (int64_t i = b; i < e; ++i) s += a[i];
+  for (int64_t i = b; i < e; ++i) s += row[i];
   part[t] = s;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    uint32_t v = t >= o ? part[t - o] : 0u;
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
     __syncthreads();
     part[t] += v;
     __syncthreads();
   }
-  uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+  uint32_t run = part[t] - s;
   for (int64_t i = b; i < e; ++i) {
-    uint32_t v = a[i];
-    a[i] = run;
+    const uint32_t v = row[i];
+    row[i] = run;
     run += v;
   }
+  if (t == 255) totals[blockIdx.x] = part[255];
 }
 
 // Stable scatter: 16 rounds of 256 consecutive keys; ranks within a wave from 8 ballots (multi-split),
@@ -185,12 +247,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout, int64_t n, int shift,
-                                                                const uint32_t* __restrict__ hist, int64_t tiles) {
+                                                                const uint32_t* __restrict__ hist, int64_t tiles,
+                                                                const uint32_t* __restrict__ totals) {
   __shared__ uint32_t offs[kRadix];
   __shared__ uint32_t cnt[4][kRadix];
   __shared__ uint32_t tot[kRadix];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  offs[tid] = hist[(int64_t)tid * tiles + blockIdx.x];
+  // digit base = exclusive scan of the digit totals
+  tot[tid] = totals[tid];
+  __syncthreads();
+  for (int o = 1; o < kRadix; o <<= 1) {
+    const uint32_t v = tid >= o ? tot[tid - o] : 0u;
+    __syncthreads();
+    tot[tid] += v;
+    __syncthreads();
+  }
+  offs[tid] = hist[(int64_t)tid * tiles + blockIdx.x] + tot[tid] - totals[tid];
+  __syncthreads();
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   int64_t base = (int64_t)blockIdx.x * kSortTile;
   for (int it = 0; it < kSortItems; ++it) {
@@ -243,8 +316,9 @@ __device__ __forceinline__ int kdelta(const uint32_t* keys, int64_t n, int64_t a
   return 32 + __clz((uint32_t)a ^ (uint32_t)b);
 }
 
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int64_t n,
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ header,
                                                 float4* __restrict__ nodes, int2* __restrict__ range) {
+  const int64_t n = header[9];  // leaves in use
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -283,37 +357,19 @@ __device__ __forceinline__ void box_store(float* p, Box b) {
   p[0] = b.lx; p[1] = b.ly; p[2] = b.lz; p[3] = b.hx; p[4] = b.hy; p[5] = b.hz;
 }
 
-// Per sorted leaf, blocks of kLeafBlock leaves: the traversal record, the leaf box in sorted order, in-block
-// prefix and suffix unions, and the block union (level 0 of the sparse table).
-//   record (64 B): rows (M_k, b_k), k = 0..2, with M = diag(1/s) L_q, b = -M mean, so that the ellipsoid-frame
-//   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
-//   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
-//   row 3 = (density, R, G, B).
-__global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(ever_prims P, const float* __restrict__ aabb,
-                                                            const float4* __restrict__ colr,
-                                                            const uint32_t* __restrict__ perm,
-                                                            float4* __restrict__ rec, float* __restrict__ sbox,
-                                                            float* __restrict__ pre, float* __restrict__ suf,
-                                                            float* __restrict__ tab0) {
+// Per sorted leaf, blocks of kLeafBlock leaves: the leaf box in sorted order, in-block prefix and suffix unions,
+// and the block union (level 0 of the sparse table). Slots beyond the L used leaves are empty boxes.
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(const uint32_t* __restrict__ header,
+                                                            const float* __restrict__ aabb,
+                                                            const uint32_t* __restrict__ order,
+                                                            float* __restrict__ sbox, float* __restrict__ pre,
+                                                            float* __restrict__ suf, float* __restrict__ tab0) {
   __shared__ float sh[6][kLeafBlock];
-  const int64_t n = P.num_prims;
+  const int64_t n = header[9];
   const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + threadIdx.x;
   Box b = box_empty();
   if (sidx < n) {
-    const uint32_t i = perm[sidx];
-    float4 q = reinterpret_cast<const float4*>(P.quats)[i];
-    f3 s = ld3(P.scales + 3 * (int64_t)i);
-    f3 m = ld3(P.means + 3 * (int64_t)i);
-    f3 c0 = qrot(mk3(1.f, 0.f, 0.f), q), c1 = qrot(mk3(0.f, 1.f, 0.f), q), c2 = qrot(mk3(0.f, 0.f, 1.f), q);
-    float M[3][3] = {{c0.x / s.x, c1.x / s.x, c2.x / s.x},
-                     {c0.y / s.y, c1.y / s.y, c2.y / s.y},
-                     {c0.z / s.z, c1.z / s.z, c2.z / s.z}};
-    for (int k = 0; k < 3; ++k) {
-      float bb = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
-      rec[4 * sidx + k] = make_float4(M[k][0], M[k][1], M[k][2], bb);
-    }
-    rec[4 * sidx + 3] = colr[i];
-    b = box_load(aabb + 6 * (int64_t)i);
+    b = box_load(aabb + 6 * (int64_t)order[sidx]);
     box_store(sbox + 6 * sidx, b);
   }
   // inclusive prefix (Hillis-Steele) over the block in LDS
@@ -375,11 +431,15 @@ __device__ __forceinline__ Box range_union(int32_t lo, int32_t hi, const float* 
   return x;
 }
 
-// Per internal node: the boxes of its two children, stored in the node (children's boxes live in the parent).
+// Per internal node: the boxes of its two children, stored in the node (children's boxes live in the parent);
+// leaf children are re-encoded as ~primitive so the traversal reaches the primitive record directly.
 __global__ __launch_bounds__(256) void k_node_boxes(float4* __restrict__ nodes, const int2* __restrict__ range,
                                                     const float* __restrict__ sbox, const float* __restrict__ pre,
                                                     const float* __restrict__ suf, const float* __restrict__ tab,
-                                                    int64_t n, int64_t nb) {
+                                                    const uint32_t* __restrict__ header,
+                                                    const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ fprim, int64_t nb) {
+  const int64_t n = header[9];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   float4* nd = nodes + 4 * i;
@@ -397,6 +457,9 @@ __global__ __launch_bounds__(256) void k_node_boxes(float4* __restrict__ nodes, 
   nd[0] = make_float4(bx[0].lx, bx[0].ly, bx[0].lz, bx[0].hx);
   nd[1] = make_float4(bx[0].hy, bx[0].hz, bx[1].lx, bx[1].ly);
   nd[2] = make_float4(bx[1].lz, bx[1].hx, bx[1].hy, bx[1].hz);
+  const int32_t e0 = ch[0] < 0 ? ~(int32_t)fprim[order[~ch[0]]] : ch[0];
+  const int32_t e1 = ch[1] < 0 ? ~(int32_t)fprim[order[~ch[1]]] : ch[1];
+  nd[3] = make_float4(__int_as_float(e0), __int_as_float(e1), 0.f, 0.f);
 }
 
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
@@ -407,9 +470,9 @@ hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
   int passes = 0;
   for (int shift = 0; shift < bits; shift += 8, ++passes) {
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, (int64_t)kRadix * tiles);
+    hipLaunchKernelGGL(k_scan_digits, dim3(kRadix), dim3(256), 0, st, hist, tiles, hist + (int64_t)kRadix * tiles);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
-                       shift, hist, tiles);
+                       shift, hist, tiles, hist + (int64_t)kRadix * tiles);
     uint32_t* t = kin; kin = kout; kout = t;
     t = vin; vin = vout; vout = t;
   }
@@ -426,6 +489,7 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   SceneLayout L = scene_layout(n);
   uint32_t* header = at<uint32_t>(ws, L.header);
   float* aabb = at<float>(ws, L.aabb);
+  uint32_t* fprim = at<uint32_t>(ws, L.fprim);
   uint32_t* k0 = at<uint32_t>(ws, L.keys0);
   uint32_t* v0 = at<uint32_t>(ws, L.vals0);
   uint32_t* k1 = at<uint32_t>(ws, L.keys1);
@@ -444,22 +508,22 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, header);
   if (n <= 0) return hipGetLastError();
   const unsigned nb256 = grid_for(n, 256);
-  hipLaunchKernelGGL(k_prim_prepass, dim3(nb256), dim3(256), 0, st, P, aabb, colr, bpart);
+  hipLaunchKernelGGL(k_prim_prepass, dim3(nb256), dim3(256), 0, st, P, rec, colr, bpart);
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, bpart, (int64_t)nb256, header);
-  hipLaunchKernelGGL(k_morton, dim3(nb256), dim3(256), 0, st, aabb, n, header, k0, v0);
-  hipError_t e = launch_sort_pairs(k0, v0, k1, v1, hist, n, 30, st);
+  hipLaunchKernelGGL(k_emit_fragments, dim3(nb256), dim3(256), 0, st, P, header, aabb, fprim, k0, v0);
+  const int64_t S = L.slots;
+  hipError_t e = launch_sort_pairs(k0, v0, k1, v1, hist, S, 32, st);
   if (e != hipSuccess) return e;
-  // sorted codes in keys0, sorted->original permutation in vals0
-  if (n > 1) hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, k0, n, nodes, range);
+  // sorted codes in keys0 (used leaves first), sorted leaf -> slot in vals0
+  hipLaunchKernelGGL(k_karras, dim3(grid_for(S, 256)), dim3(256), 0, st, k0, header, nodes, range);
   const int64_t nb = L.leaf_blocks;
-  hipLaunchKernelGGL(k_leaf_blocks, dim3((unsigned)nb), dim3(kLeafBlock), 0, st, P, aabb, colr, v0, rec, sbox, pre,
-                     suf, tab);
+  hipLaunchKernelGGL(k_leaf_blocks, dim3((unsigned)nb), dim3(kLeafBlock), 0, st, header, aabb, v0, sbox, pre, suf,
+                     tab);
   for (int k = 1; k < L.tab_levels; ++k)
     hipLaunchKernelGGL(k_table_level, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab + (int64_t)(k - 1) * 6 * nb,
                        tab + (int64_t)k * 6 * nb, nb, (int64_t)1 << (k - 1));
-  if (n > 1)
-    hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, 256)), dim3(256), 0, st, nodes, range, sbox, pre, suf, tab,
-                       n, nb);
+  hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(S, 256)), dim3(256), 0, st, nodes, range, sbox, pre, suf, tab,
+                     header, v0, fprim, nb);
   return hipGetLastError();
 }
 

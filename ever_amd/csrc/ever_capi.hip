@@ -22,8 +22,8 @@ int hip_status(hipError_t e, const char* where) {
 int check_prims(const ever_prims* P) {
   if (!P) return fail(EVER_ERR_INVALID, "prims is null");
   if (P->num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
-  if (P->num_prims >= (int64_t(1) << 30))
-    return fail(EVER_ERR_INVALID, "num_prims must be < 2^30 (event ids are 2*prim+face in int32)");
+  if (P->num_prims >= (int64_t(1) << 28))
+    return fail(EVER_ERR_INVALID, "num_prims must be < 2^28 (4 leaf fragments per primitive, int32 leaf indices)");
   if (P->num_prims > 0 && (!P->means || !P->scales || !P->quats || !P->densities || !P->features))
     return fail(EVER_ERR_INVALID, "a primitive array is null");
   if (P->feature_size < 1) return fail(EVER_ERR_INVALID, "feature_size must be >= 1");
@@ -67,9 +67,9 @@ const char* ever_last_error(void) { return g_last_error.c_str(); }
 int ever_scene_layout(int64_t num_prims, int64_t* o) {
   if (!o) return fail(EVER_ERR_INVALID, "offsets is null");
   ever::SceneLayout L = ever::scene_layout(num_prims > 0 ? num_prims : 0);
-  const size_t v[18] = {L.header, L.aabb, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes, L.range,
+  const size_t v[19] = {L.header, L.aabb, L.fprim, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes, L.range,
                         L.sbox, L.pre, L.suf, L.tab, L.rec, L.colr, L.cand, L.bpart, L.total};
-  for (int i = 0; i < 18; ++i) o[i] = (int64_t)v[i];
+  for (int i = 0; i < 19; ++i) o[i] = (int64_t)v[i];
   return EVER_OK;
 }
 

@@ -22,7 +22,6 @@ constexpr int kSpillDepth = 32;   // global spill entries per ray (depth bound o
 struct FwdArgs {
   const float4* rec;
   const float4* nodes;
-  const uint32_t* perm;
   const float4* colr;
   const int32_t* cand;
   const uint32_t* header;
@@ -49,18 +48,27 @@ struct FwdArgs {
   const uint32_t* order;  // processing order: thread slot -> ray index
 };
 
-// kern_prefilter (initialize_density.cu:69-105): primitives whose AABB lies within tmin of ray 0's origin.
-__global__ __launch_bounds__(256) void k_prefilter(const float* __restrict__ aabb, int64_t n,
-                                                   const float* __restrict__ rayo, float tmin,
+// kern_prefilter (initialize_density.cu:69-105): primitives whose AABB (create_aabbs.cu:77-114) lies within tmin
+// of ray 0's origin.
+__global__ __launch_bounds__(256) void k_prefilter(ever_prims P, const float* __restrict__ rayo, float tmin,
                                                    int32_t* __restrict__ cand, uint32_t* __restrict__ header) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* b = aabb + 6 * i;
-  float o[3] = {rayo[0], rayo[1], rayo[2]};
+  if (i >= P.num_prims) return;
+  const float4 q = reinterpret_cast<const float4*>(P.quats)[i];
+  const float qn = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+  const float r = q.x * qn, x = q.y * qn, y = q.z * qn, z = q.w * qn;
+  const float Rm[3][3] = {{1.f - 2.f * (y * y + z * z), 2.f * (x * y - r * z), 2.f * (x * z + r * y)},
+                          {2.f * (x * y + r * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z - r * x)},
+                          {2.f * (x * z - r * y), 2.f * (y * z + r * x), 1.f - 2.f * (x * x + y * y)}};
+  const float sv[3] = {P.scales[3 * i], P.scales[3 * i + 1], P.scales[3 * i + 2]};
+  const float o[3] = {rayo[0], rayo[1], rayo[2]};
   float dmin = 0.f;
   for (int k = 0; k < 3; ++k) {
-    if (o[k] < b[k]) dmin += (o[k] - b[k]) * (o[k] - b[k]);
-    else if (o[k] > b[3 + k]) dmin += (o[k] - b[3 + k]) * (o[k] - b[3 + k]);
+    const float e = sqrtf(Rm[k][0] * Rm[k][0] * sv[0] * sv[0] + Rm[k][1] * Rm[k][1] * sv[1] * sv[1] +
+                          Rm[k][2] * Rm[k][2] * sv[2] * sv[2]) * (1.f + 1e-6f);
+    const float m = P.means[3 * i + k];
+    if (o[k] < m - e) dmin += (o[k] - (m - e)) * (o[k] - (m - e));
+    else if (o[k] > m + e) dmin += (o[k] - (m + e)) * (o[k] - (m + e));
   }
   if (dmin <= tmin * tmin) {
     uint32_t pos = atomicAdd(header + 8, 1u);
@@ -125,6 +133,11 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 template <int K>
 __device__ __forceinline__ void kbuf_insert(uint64_t (&bk)[K], uint64_t v) {
   if (!(v < bk[K - 1])) return;
+  // the fragments of one primitive report identical events: keep one
+  bool dup = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) dup |= (bk[k] == v);
+  if (dup) return;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool sw = v < bk[k];
@@ -233,7 +246,8 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   float cur_t = 0.f;
   uint64_t cursor = pack_event(0.f, 0xffffffffu);
   int iter = 0;
-  const int32_t root = A.n == 1 ? ~0 : 0;
+  const int64_t nleaves = A.header[9];
+  const int32_t root = nleaves == 1 ? ~0 : 0;
   int32_t* spill = A.spill + slot * kSpillDepth;
 
   while (logT < (double)kLogCutoff && iter < A.max_iters) {
@@ -244,7 +258,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
     int sp = 0;
     int32_t node = root;
     int64_t steps = 0;
-    const int64_t max_steps = 2 * A.n + 2;  // a tree visits each node at most once per traversal
+    const int64_t max_steps = 2 * nleaves + 2;  // a tree visits each node at most once per traversal
     while (++steps <= max_steps) {
       if (node >= 0) {
         const float4* nd = A.nodes + 4 * (int64_t)node;
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       const bool entry = (key & 1u) == 0u;
       const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
       const float4 Dr = A.rec[4 * (int64_t)s + 3];
-      const uint32_t orig = A.perm[s];
+      const uint32_t orig = s;  // records are in original primitive order
       const f3 mu = ld3(A.means + 3 * (int64_t)orig);
       const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
       const float hgt = entry ? Dr.x : -Dr.x;
@@ -385,8 +399,8 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
     hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, O.touch_count, P.num_prims);
   if (R.num_rays <= 0) return hipGetLastError();
   if (P.num_prims > 0)
-    hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, at<float>(ws, L.aabb),
-                       P.num_prims, R.origins, R.tmin, cand, header);
+    hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, P, R.origins, R.tmin, cand,
+                       header);
   // processing order
   ForwardLayout FL = forward_layout(R.num_rays);
   uint32_t* k0 = at<uint32_t>(O.workspace, FL.keys0);
@@ -404,7 +418,6 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   FwdArgs A;
   A.rec = at<float4>(ws, L.rec);
   A.nodes = at<float4>(ws, L.nodes);
-  A.perm = at<uint32_t>(ws, L.vals0);
   A.colr = at<float4>(ws, L.colr);
   A.cand = cand;
   A.header = header;

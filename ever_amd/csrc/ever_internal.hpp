@@ -17,36 +17,43 @@ constexpr int kRadix = 256;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-constexpr int kLeafBlock = 256;  // leaves per block of the range-union refit
+constexpr int kLeafBlock = 256;   // leaves per block of the range-union refit
+constexpr int kMaxFragments = 4;  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
 
 // Scene workspace: everything the LBVH build writes and the tracers read. All offsets 256-byte aligned.
-//   header   : uint32[16]  (0..5 centroid bounds as ordered ints, 8 = start-inside candidate count)
-//   aabb     : float[N*6]  per-primitive box (original order)
-//   keys0/1, vals0/1 : uint32[N] radix-sort ping-pong (vals0 ends as the sorted->original permutation)
-//   hist     : uint32[256 * tiles]
-//   nodes    : float4[(N-1) * 4] BVH2 nodes, children boxes stored in the parent (64 B per node)
-//   range    : int2[N-1] leaf range of each internal node
-//   sbox/pre/suf : float[N*6] leaf boxes in sorted order, in-block prefix and suffix unions
+// Leaves are fragments: each primitive owns kMaxFragments slots (S = kMaxFragments * N); unused slots carry the
+// sort key 0xffffffff and end behind the L used leaves (L is counted on the device, header[9]).
+//   header   : uint32[16]  (0..5 centroid bounds as ordered ints, 8 = start-inside candidate count, 9 = L)
+//   aabb     : float[S*6]  fragment boxes (slot order)
+//   fprim    : uint32[S]   slot -> primitive
+//   keys0/1, vals0/1 : uint32[S] radix-sort ping-pong (vals0 ends as sorted leaf -> slot)
+//   hist     : uint32[256 * tiles + 256] (digit-major tile counts, then digit totals)
+//   nodes    : float4[(S-1) * 4] BVH2 nodes, children boxes stored in the parent (64 B per node); a leaf child is
+//              encoded as ~primitive
+//   range    : int2[S-1] leaf range of each internal node
+//   sbox/pre/suf : float[S*6] leaf boxes in sorted order, in-block prefix and suffix unions
 //   tab      : float[levels * blocks * 6] sparse table of block unions
-//   rec      : float4[N * 4] per-primitive traversal record in LBVH order (64 B)
+//   rec      : float4[N * 4] per-primitive traversal record, original order (64 B)
 //   colr     : float4[N] (density, R, G, B) in original order
 //   cand     : int32[N] start-inside candidates
 //   bpart    : float[6 * ceil(N/256)] per-block centroid bounds
 struct SceneLayout {
-  int64_t n = 0, tiles = 0, leaf_blocks = 0;
+  int64_t n = 0, slots = 0, tiles = 0, leaf_blocks = 0;
   int tab_levels = 1;
-  size_t header, aabb, keys0, vals0, keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand, bpart,
-      total;
+  size_t header, aabb, fprim, keys0, vals0, keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand,
+      bpart, total;
 };
 
 inline SceneLayout scene_layout(int64_t n) {
   SceneLayout L;
   L.n = n;
-  L.tiles = (n + kSortTile - 1) / kSortTile;
-  L.leaf_blocks = (n + kLeafBlock - 1) / kLeafBlock;
+  L.slots = (int64_t)kMaxFragments * n;
+  const int64_t S = L.slots;
+  L.tiles = (S + kSortTile - 1) / kSortTile;
+  L.leaf_blocks = (S + kLeafBlock - 1) / kLeafBlock;
   L.tab_levels = 1;
   while ((int64_t(1) << L.tab_levels) <= L.leaf_blocks) ++L.tab_levels;
-  int64_t ni = n > 1 ? n - 1 : 1;
+  const int64_t ni = S > 1 ? S - 1 : 1;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -54,17 +61,18 @@ inline SceneLayout scene_layout(int64_t n) {
     return o;
   };
   L.header = take(64);
-  L.aabb = take(sizeof(float) * 6 * n);
-  L.keys0 = take(4 * n);
-  L.vals0 = take(4 * n);
-  L.keys1 = take(4 * n);
-  L.vals1 = take(4 * n);
-  L.hist = take(4 * (size_t)kRadix * (L.tiles > 0 ? L.tiles : 1));
+  L.aabb = take(sizeof(float) * 6 * S);
+  L.fprim = take(4 * S);
+  L.keys0 = take(4 * S);
+  L.vals0 = take(4 * S);
+  L.keys1 = take(4 * S);
+  L.vals1 = take(4 * S);
+  L.hist = take(4 * (size_t)kRadix * ((L.tiles > 0 ? L.tiles : 1) + 1));  // + digit totals
   L.nodes = take(64 * ni);
   L.range = take(8 * ni);
-  L.sbox = take(24 * n);
-  L.pre = take(24 * n);
-  L.suf = take(24 * n);
+  L.sbox = take(24 * S);
+  L.pre = take(24 * S);
+  L.suf = take(24 * S);
   L.tab = take(24 * (size_t)L.tab_levels * (L.leaf_blocks > 0 ? L.leaf_blocks : 1));
   L.rec = take(64 * n);
   L.colr = take(16 * n);
@@ -78,7 +86,7 @@ inline SceneLayout scene_layout(int64_t n) {
 // colour 3, means2D 2) and inverted world-to-camera matrices.
 // ... and the adjoint of the start-inside density per ray (used when the caller passes no buffer for it).
 struct BackwardLayout {
-  size_t acc, inv, dinit, total;
+  size_t acc, inv, dinit, grgb, total;
 };
 inline BackwardLayout backward_layout(int64_t n, int64_t nr, int64_t nw) {
   BackwardLayout L;
@@ -89,6 +97,8 @@ inline BackwardLayout backward_layout(int64_t n, int64_t nr, int64_t nw) {
   off = align_up(off + 64 * (size_t)(nw > 0 ? nw : 1), 256);
   L.dinit = off;
   off = align_up(off + 16 * (size_t)(nr > 0 ? nr : 1), 256);
+  L.grgb = off;
+  off = align_up(off + 16 * (size_t)(n > 0 ? n : 1), 256);
   L.total = off;
   return L;
 }
@@ -135,7 +145,7 @@ inline ForwardLayout forward_layout(int64_t nr) {
   L.vals0 = take(4 * (size_t)r);
   L.keys1 = take(4 * (size_t)r);
   L.vals1 = take(4 * (size_t)r);
-  L.hist = take(4 * (size_t)kRadix * L.tiles);
+  L.hist = take(4 * (size_t)kRadix * (L.tiles + 1));  // + digit totals
   L.total = off;
   return L;
 }
