@@ -103,8 +103,8 @@ def stream_ptr(device=None):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
-LAYOUT_NAMES = ("header", "aabb", "fprim", "keys0", "vals0", "keys1", "vals1", "hist", "nodes", "range", "sbox",
-                "pre", "suf", "tab", "rec", "colr", "cand", "bpart", "total")
+LAYOUT_NAMES = ("header", "aabb", "fprim", "keys0", "vals0", "keys1", "vals1", "hist", "nodes", "nodes4", "range",
+                "sbox", "pre", "suf", "tab", "rec", "colr", "cand", "bpart", "total")
 
 
 def scene_layout(n):

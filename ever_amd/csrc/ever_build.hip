@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
   int64_t lo = i < j ? i : j, hi = i < j ? j : i;
   const int32_t left = (lo == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
   const int32_t right = (hi == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
-  nodes[4 * i + 3] = make_float4(__int_as_float(left), __int_as_float(right), 0.f, 0.f);
+  nodes[i] = make_float4(__int_as_float(left), __int_as_float(right), 0.f, 0.f);
   range[i] = make_int2((int32_t)lo, (int32_t)hi);
 }
 
@@ -431,35 +431,56 @@ __device__ __forceinline__ Box range_union(int32_t lo, int32_t hi, const float* 
   return x;
 }
 
-// Per internal node: the boxes of its two children, stored in the node (children's boxes live in the parent);
-// leaf children are re-encoded as ~primitive so the traversal reaches the primitive record directly.
-__global__ __launch_bounds__(256) void k_node_boxes(float4* __restrict__ nodes, const int2* __restrict__ range,
-                                                    const float* __restrict__ sbox, const float* __restrict__ pre,
-                                                    const float* __restrict__ suf, const float* __restrict__ tab,
-                                                    const uint32_t* __restrict__ header,
-                                                    const uint32_t* __restrict__ order,
-                                                    const uint32_t* __restrict__ fprim, int64_t nb) {
+// 4-wide node records: the record of binary node i describes up to four children (its children, with an internal
+// child replaced by that child's two children), boxes stored per axis (SoA) so one record is 128 B:
+//   f4[0] = lo.x of children 0..3, f4[1] = hi.x, f4[2] = lo.y, f4[3] = hi.y, f4[4] = lo.z, f4[5] = hi.z,
+//   f4[6] = child refs (internal: binary node index of the child's record; leaf: ~primitive; empty: kEmptyChild)
+// Only records reachable from the root (binary node 0) are visited.
+__device__ __forceinline__ Box child_box(int32_t c, const int2* range, const float* sbox, const float* pre,
+                                         const float* suf, const float* tab, int64_t nb) {
+  if (c < 0) return box_load(sbox + 6 * (int64_t)(~c));
+  const int2 r = range[c];
+  return range_union(r.x, r.y, sbox, pre, suf, tab, nb);
+}
+
+__global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2, float4* __restrict__ nodes4,
+                                               const int2* __restrict__ range, const float* __restrict__ sbox,
+                                               const float* __restrict__ pre, const float* __restrict__ suf,
+                                               const float* __restrict__ tab, const uint32_t* __restrict__ header,
+                                               const uint32_t* __restrict__ order, const uint32_t* __restrict__ fprim,
+                                               int64_t nb) {
   const int64_t n = header[9];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  float4* nd = nodes + 4 * i;
-  const float4 e = nd[3];
-  Box bx[2];
+  const float4 e = nodes2[i];
   const int32_t ch[2] = {__float_as_int(e.x), __float_as_int(e.y)};
+  int32_t ref[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+  Box bx[4] = {box_empty(), box_empty(), box_empty(), box_empty()};
+  int m = 0;
   for (int c = 0; c < 2; ++c) {
     if (ch[c] < 0) {
-      bx[c] = box_load(sbox + 6 * (int64_t)(~ch[c]));
+      ref[m] = ~(int32_t)fprim[order[~ch[c]]];
+      bx[m] = box_load(sbox + 6 * (int64_t)(~ch[c]));
+      ++m;
     } else {
-      const int2 r = range[ch[c]];
-      bx[c] = range_union(r.x, r.y, sbox, pre, suf, tab, nb);
+      const float4 ge = nodes2[ch[c]];
+      const int32_t g[2] = {__float_as_int(ge.x), __float_as_int(ge.y)};
+      for (int k = 0; k < 2; ++k) {
+        bx[m] = child_box(g[k], range, sbox, pre, suf, tab, nb);
+        ref[m] = g[k] < 0 ? ~(int32_t)fprim[order[~g[k]]] : g[k];
+        ++m;
+      }
     }
   }
-  nd[0] = make_float4(bx[0].lx, bx[0].ly, bx[0].lz, bx[0].hx);
-  nd[1] = make_float4(bx[0].hy, bx[0].hz, bx[1].lx, bx[1].ly);
-  nd[2] = make_float4(bx[1].lz, bx[1].hx, bx[1].hy, bx[1].hz);
-  const int32_t e0 = ch[0] < 0 ? ~(int32_t)fprim[order[~ch[0]]] : ch[0];
-  const int32_t e1 = ch[1] < 0 ? ~(int32_t)fprim[order[~ch[1]]] : ch[1];
-  nd[3] = make_float4(__int_as_float(e0), __int_as_float(e1), 0.f, 0.f);
+  float4* nd = nodes4 + 8 * i;
+  nd[0] = make_float4(bx[0].lx, bx[1].lx, bx[2].lx, bx[3].lx);
+  nd[1] = make_float4(bx[0].hx, bx[1].hx, bx[2].hx, bx[3].hx);
+  nd[2] = make_float4(bx[0].ly, bx[1].ly, bx[2].ly, bx[3].ly);
+  nd[3] = make_float4(bx[0].hy, bx[1].hy, bx[2].hy, bx[3].hy);
+  nd[4] = make_float4(bx[0].lz, bx[1].lz, bx[2].lz, bx[3].lz);
+  nd[5] = make_float4(bx[0].hz, bx[1].hz, bx[2].hz, bx[3].hz);
+  nd[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  nd[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
@@ -496,6 +517,7 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   uint32_t* v1 = at<uint32_t>(ws, L.vals1);
   uint32_t* hist = at<uint32_t>(ws, L.hist);
   float4* nodes = at<float4>(ws, L.nodes);
+  float4* nodes4 = at<float4>(ws, L.nodes4);
   int2* range = at<int2>(ws, L.range);
   float* sbox = at<float>(ws, L.sbox);
   float* pre = at<float>(ws, L.pre);
@@ -522,7 +544,7 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   for (int k = 1; k < L.tab_levels; ++k)
     hipLaunchKernelGGL(k_table_level, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab + (int64_t)(k - 1) * 6 * nb,
                        tab + (int64_t)k * 6 * nb, nb, (int64_t)1 << (k - 1));
-  hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(S, 256)), dim3(256), 0, st, nodes, range, sbox, pre, suf, tab,
+  hipLaunchKernelGGL(k_node4, dim3(grid_for(S, 256)), dim3(256), 0, st, nodes, nodes4, range, sbox, pre, suf, tab,
                      header, v0, fprim, nb);
   return hipGetLastError();
 }

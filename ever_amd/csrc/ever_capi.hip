@@ -67,9 +67,9 @@ const char* ever_last_error(void) { return g_last_error.c_str(); }
 int ever_scene_layout(int64_t num_prims, int64_t* o) {
   if (!o) return fail(EVER_ERR_INVALID, "offsets is null");
   ever::SceneLayout L = ever::scene_layout(num_prims > 0 ? num_prims : 0);
-  const size_t v[19] = {L.header, L.aabb, L.fprim, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes, L.range,
-                        L.sbox, L.pre, L.suf, L.tab, L.rec, L.colr, L.cand, L.bpart, L.total};
-  for (int i = 0; i < 19; ++i) o[i] = (int64_t)v[i];
+  const size_t v[20] = {L.header, L.aabb, L.fprim, L.keys0, L.vals0, L.keys1, L.vals1, L.hist, L.nodes, L.nodes4,
+                        L.range, L.sbox, L.pre, L.suf, L.tab, L.rec, L.colr, L.cand, L.bpart, L.total};
+  for (int i = 0; i < 20; ++i) o[i] = (int64_t)v[i];
   return EVER_OK;
 }
 

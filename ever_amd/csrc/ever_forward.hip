@@ -17,11 +17,11 @@ namespace ever {
 
 constexpr int kFwdBlock = 64;
 constexpr int kLdsDepth = 32;     // LDS stack entries per lane
-constexpr int kSpillDepth = 32;   // global spill entries per ray (depth bound of a 30-bit LBVH: 30 + 32)
+constexpr int kSpillDepth = kSpillPerRay;  // global spill entries per ray
 
 struct FwdArgs {
   const float4* rec;
-  const float4* nodes;
+  const float4* nodes4;
   const float4* colr;
   const int32_t* cand;
   const uint32_t* header;
@@ -261,24 +261,45 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
     const int64_t max_steps = 2 * nleaves + 2;  // a tree visits each node at most once per traversal
     while (++steps <= max_steps) {
       if (node >= 0) {
-        const float4* nd = A.nodes + 4 * (int64_t)node;
-        const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+        // 4-wide record: slab test of up to four children, nearest first
+        const float4* nd = A.nodes4 + 8 * (int64_t)node;
+        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cf = nd[6];
         const float t_hi = fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
-        float tn0, tn1;
-        const bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, cur_t, t_hi, tn0);
-        const bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, cur_t, t_hi, tn1);
-        const int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-        if (h0 && h1 && sp < kLdsDepth + kSpillDepth) {  // depth of a 30-bit LBVH never exceeds 62
-          const bool first0 = tn0 <= tn1;
-          const int32_t farc = first0 ? c1 : c0;
-          if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = farc;
-          else spill[sp - kLdsDepth] = farc;
-          ++sp;
-          node = first0 ? c0 : c1;
+        float d[4];
+        int32_t c[4] = {__float_as_int(cf.x), __float_as_int(cf.y), __float_as_int(cf.z), __float_as_int(cf.w)};
+        const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+        int nhit = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          float tn;
+          // unused slots (kEmptyChild) never hit, whatever their stored box
+          const bool h = slab(lxs[k], lys[k], lzs[k], hxs[k], hys[k], hzs[k], o, inv, cur_t, t_hi, tn) &&
+                         c[k] != kEmptyChild;
+          d[k] = h ? tn : INFINITY;
+          nhit += h ? 1 : 0;
+        }
+        if (nhit > 0) {
+          // sorting network on (d, c), ascending
+#define EVER_CSWAP(a, b)                                      \
+  if (d[b] < d[a]) {                                          \
+    const float td = d[a]; d[a] = d[b]; d[b] = td;            \
+    const int32_t tc = c[a]; c[a] = c[b]; c[b] = tc;          \
+  }
+          EVER_CSWAP(0, 1) EVER_CSWAP(2, 3) EVER_CSWAP(0, 2) EVER_CSWAP(1, 3) EVER_CSWAP(1, 2)
+#undef EVER_CSWAP
+#pragma unroll
+          for (int k = 3; k >= 1; --k) {
+            if (k < nhit && sp < kLdsDepth + kSpillDepth) {  // bounded by 3 x (LBVH depth / 2)
+              if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = c[k];
+              else spill[sp - kLdsDepth] = c[k];
+              ++sp;
+            }
+          }
+          node = c[0];
           continue;
         }
-        if (h0) { node = c0; continue; }
-        if (h1) { node = c1; continue; }
       } else {
         // leaf: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame
         const uint32_t s = (uint32_t)(~node);
@@ -417,7 +438,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, v0, O.ray_order, R.num_rays);
   FwdArgs A;
   A.rec = at<float4>(ws, L.rec);
-  A.nodes = at<float4>(ws, L.nodes);
+  A.nodes4 = at<float4>(ws, L.nodes4);
   A.colr = at<float4>(ws, L.colr);
   A.cand = cand;
   A.header = header;

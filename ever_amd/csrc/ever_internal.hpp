@@ -18,7 +18,8 @@ constexpr int kRadix = 256;
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr int kLeafBlock = 256;   // leaves per block of the range-union refit
-constexpr int kMaxFragments = 4;  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
+constexpr int kMaxFragments = 4;
+constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 4-wide node  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
 
 // Scene workspace: everything the LBVH build writes and the tracers read. All offsets 256-byte aligned.
 // Leaves are fragments: each primitive owns kMaxFragments slots (S = kMaxFragments * N); unused slots carry the
@@ -28,8 +29,8 @@ constexpr int kMaxFragments = 4;  // leaf fragments per primitive (elongated ell
 //   fprim    : uint32[S]   slot -> primitive
 //   keys0/1, vals0/1 : uint32[S] radix-sort ping-pong (vals0 ends as sorted leaf -> slot)
 //   hist     : uint32[256 * tiles + 256] (digit-major tile counts, then digit totals)
-//   nodes    : float4[(S-1) * 4] BVH2 nodes, children boxes stored in the parent (64 B per node); a leaf child is
-//              encoded as ~primitive
+//   nodes    : float4[S-1] binary hierarchy: child indices (leaf child = ~sorted leaf)
+//   nodes4   : float4[(S-1) * 8] 4-wide records (128 B): children's boxes per axis + child refs (leaf = ~primitive)
 //   range    : int2[S-1] leaf range of each internal node
 //   sbox/pre/suf : float[S*6] leaf boxes in sorted order, in-block prefix and suffix unions
 //   tab      : float[levels * blocks * 6] sparse table of block unions
@@ -40,7 +41,7 @@ constexpr int kMaxFragments = 4;  // leaf fragments per primitive (elongated ell
 struct SceneLayout {
   int64_t n = 0, slots = 0, tiles = 0, leaf_blocks = 0;
   int tab_levels = 1;
-  size_t header, aabb, fprim, keys0, vals0, keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand,
+  size_t header, aabb, fprim, keys0, vals0, keys1, vals1, hist, nodes, nodes4, range, sbox, pre, suf, tab, rec, colr, cand,
       bpart, total;
 };
 
@@ -68,7 +69,8 @@ inline SceneLayout scene_layout(int64_t n) {
   L.keys1 = take(4 * S);
   L.vals1 = take(4 * S);
   L.hist = take(4 * (size_t)kRadix * ((L.tiles > 0 ? L.tiles : 1) + 1));  // + digit totals
-  L.nodes = take(64 * ni);
+  L.nodes = take(16 * ni);
+  L.nodes4 = take(128 * ni);
   L.range = take(8 * ni);
   L.sbox = take(24 * S);
   L.pre = take(24 * S);
@@ -122,7 +124,7 @@ hipError_t launch_eval_sh(const float* means, const float* feat, const float* or
 hipError_t launch_eval_sh_bwd(const float* means, const float* feat, const float* origin, const float* drgb,
                               int64_t n, int d, int deg, float* dfeat, hipStream_t s);
 
-constexpr int kSpillPerRay = 32;  // int32 traversal-stack spill entries per ray
+constexpr int kSpillPerRay = 64;  // int32 traversal-stack spill entries per ray (4-wide stack bound: 3 x 26)
 constexpr int64_t kMinRaysToSort = 65536;  // below this the rays are traced in the caller's order
 
 // Forward workspace: traversal-stack spill per ray, and the ray sort (30-bit Morton key of a point on each ray).

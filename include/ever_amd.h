@@ -109,8 +109,8 @@ int ever_struct_sizes(int64_t* out4);
 const char* ever_last_error(void);
 
 /* Byte offsets of the scene workspace regions, for inspection by tests (order: header, aabb, fprim, keys0,
- * vals0, keys1, vals1, hist, nodes, range, sbox, pre, suf, tab, rec, colr, cand, bpart, total). */
-int ever_scene_layout(int64_t num_prims, int64_t* offsets19);
+ * vals0, keys1, vals1, hist, nodes, nodes4, range, sbox, pre, suf, tab, rec, colr, cand, bpart, total). */
+int ever_scene_layout(int64_t num_prims, int64_t* offsets20);
 
 /* Scene workspace: LBVH nodes, sort buffers, per-primitive records. Lives from build to backward. */
 size_t ever_scene_workspace_bytes(int64_t num_prims);
