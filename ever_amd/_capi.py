@@ -28,7 +28,8 @@ class Prims(ctypes.Structure):
 
 class Rays(ctypes.Structure):
     _fields_ = [("origins", _vp), ("directions", _vp), ("num_rays", _i64), ("tmin", ctypes.c_float),
-                ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32)]
+                ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32), ("traversal", _i32),
+                ("reserved", _i32)]
 
 
 class ForwardOut(ctypes.Structure):

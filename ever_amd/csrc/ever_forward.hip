@@ -18,6 +18,7 @@ namespace ever {
 constexpr int kFwdBlock = 64;
 constexpr int kLdsDepth = 32;     // LDS stack entries per lane
 constexpr int kSpillDepth = kSpillPerRay;  // global spill entries per ray
+constexpr int kWaveBlock = 64;   // one wave per block in the per-wave traversal
 
 struct FwdArgs {
   const float4* rec;
@@ -151,6 +152,25 @@ __device__ __forceinline__ uint64_t pack_event(float t, uint32_t key) {
   return ((uint64_t)__float_as_uint(t + 0.f) << 32) | key;
 }
 
+constexpr uint64_t kNoEvent = ((uint64_t)0x7f800000u << 32) | 0xffffffffu;  // pack_event(kEmptyT, ~0)
+
+// Inserts up to two events without branches: an event that must not enter is passed as kNoEvent, which equals
+// the empty slots and moves nothing. Same result as kbuf_insert of each event in turn.
+template <int K>
+__device__ __forceinline__ void kbuf_insert_pred(uint64_t (&bk)[K], uint64_t v) {
+  bool dup = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) dup |= (bk[k] == v);
+  v = dup ? kNoEvent : v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool sw = v < bk[k];
+    const uint64_t old = bk[k];
+    bk[k] = sw ? v : old;
+    v = sw ? old : v;
+  }
+}
+
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
                                      float t_lo, float t_hi, float& t_near) {
   float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
@@ -183,77 +203,222 @@ __device__ __forceinline__ double event_depth_fp64(float4 RA, float4 RB, float4 
   return entry ? fmin(x0, x1) : fmax(x0, x1);
 }
 
+// Per-ray state of the forward pass: the ray in fp32 (traversal, leaf tests) and FP64 (event depths), the
+// integrator state and the (t, key) cursor.
+struct RayState {
+  int64_t r;
+  f3 o, d, inv, oi;
+  double odx, ody, odz, ddx, ddy, ddz;
+  double D0, D1, D2, D3, t_state, logT;
+  float C0, C1, C2, csx, csy, dpx, dpy;
+  float cur_t;
+  uint64_t cursor;
+  int iter;
+};
+
+__device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState& S) {
+  S.r = r;
+  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
+  S.d = l2n(rdraw);
+  S.o = ro + A.tmin * rdraw;  // shaders.slang:178-179: origin shifted along the raw direction
+  S.inv = mk3(fabsf(S.d.x) > 1e-20f ? 1.f / S.d.x : copysignf(1e20f, S.d.x),
+              fabsf(S.d.y) > 1e-20f ? 1.f / S.d.y : copysignf(1e20f, S.d.y),
+              fabsf(S.d.z) > 1e-20f ? 1.f / S.d.z : copysignf(1e20f, S.d.z));
+  S.oi = mk3(S.o.x * S.inv.x, S.o.y * S.inv.y, S.o.z * S.inv.z);  // slab distances as fma(plane, inv, -o*inv)
+  S.odx = (double)ro.x + (double)A.tmin * rdraw.x;
+  S.ody = (double)ro.y + (double)A.tmin * rdraw.y;
+  S.odz = (double)ro.z + (double)A.tmin * rdraw.z;
+  S.ddx = rdraw.x;
+  S.ddy = rdraw.y;
+  S.ddz = rdraw.z;
+  {
+    const double nn = 1.0 / sqrt(S.ddx * S.ddx + S.ddy * S.ddy + S.ddz * S.ddz);
+    S.ddx *= nn;
+    S.ddy *= nn;
+    S.ddz *= nn;
+  }
+  // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
+  double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
+  const int32_t ncand = (int32_t)A.header[8];
+  const f3 p = ro + A.tmin * S.d;
+  for (int32_t j = 0; j < ncand; ++j) {
+    const int64_t i = A.cand[j];
+    float4 q = reinterpret_cast<const float4*>(A.quats)[i];
+    float qn = 1.f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    q = make_float4(q.x * qn, q.y * qn, q.z * qn, q.w * qn);
+    f3 v = qrot(p - ld3(A.means + 3 * i), q);
+    f3 sc = ld3(A.scales + 3 * i);
+    f3 tr = mk3(v.x / sc.x, v.y / sc.y, v.z / sc.z);
+    if (dot3(tr, tr) <= 1.f) {
+      float den = A.dens[i];
+      float c0, c1, c2;
+      if (A.color_mode == EVER_COLOR_SH) {
+        float4 cr = A.colr[i];
+        c0 = cr.y; c1 = cr.z; c2 = cr.w;
+      } else {  // the reference reads the features at flat index prim*3 + c (initialize_density.cu:140-142)
+        c0 = A.feat[3 * i + 0] * kShC0 + 0.5f;
+        c1 = A.feat[3 * i + 1] * kShC0 + 0.5f;
+        c2 = A.feat[3 * i + 2] * kShC0 + 0.5f;
+      }
+      D0 += den;
+      D1 += den * c0;
+      D2 += den * c1;
+      D3 += den * c2;
+    }
+  }
+  reinterpret_cast<float4*>(A.init_drgb)[r] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
+  S.D0 = D0;
+  S.D1 = D1;
+  S.D2 = D2;
+  S.D3 = D3;
+  S.C0 = S.C1 = S.C2 = S.csx = S.csy = S.dpx = S.dpy = 0.f;
+  S.t_state = 0.0;
+  S.logT = 0.0;
+  // cursor over (t, key): initially nothing at t == 0 (the reference requires t > t_cur = 0)
+  S.cur_t = 0.f;
+  S.cursor = pack_event(0.f, 0xffffffffu);
+  S.iter = 0;
+}
+
+__device__ __forceinline__ bool ray_alive(const FwdArgs& A, const RayState& S) {
+  return S.logT < (double)kLogCutoff && S.iter < A.max_iters;
+}
+
+// Leaf test of one primitive: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame, then the
+// intersection program (shaders.slang:96-106) and the any-hit insertion of entry and exit (:138-159), with the
+// (t, key) cursor in place of t > t_cur.
+// The roots are written with explicit fused multiply-adds and no other contraction, so both tree walks (records
+// in vector or in scalar registers) produce bit-identical depths and therefore the same event order.
+__device__ __forceinline__ float dot3_fma(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+
+template <int K>
+__device__ __forceinline__ void leaf_test(const FwdArgs& A, const RayState& S, float4 RA, float4 RB, float4 RC,
+                                          uint32_t s, uint64_t (&bk)[K]) {
+#pragma clang fp contract(off)
+  const f3 o = S.o, d = S.d;
+  const f3 ocn = mk3(fmaf(RA.z, o.z, fmaf(RA.y, o.y, fmaf(RA.x, o.x, RA.w))),
+                     fmaf(RB.z, o.z, fmaf(RB.y, o.y, fmaf(RB.x, o.x, RB.w))),
+                     fmaf(RC.z, o.z, fmaf(RC.y, o.y, fmaf(RC.x, o.x, RC.w))));
+  const f3 rdn = mk3(fmaf(RA.z, d.z, fmaf(RA.y, d.y, RA.x * d.x)), fmaf(RB.z, d.z, fmaf(RB.y, d.y, RB.x * d.x)),
+                     fmaf(RC.z, d.z, fmaf(RC.y, d.y, RC.x * d.x)));
+  const float aa = dot3_fma(rdn, rdn);
+  const float bp = -dot3_fma(ocn, rdn);
+  const float ba = bp / aa;
+  const f3 l = mk3(fmaf(ba, rdn.x, ocn.x), fmaf(ba, rdn.y, ocn.y), fmaf(ba, rdn.z, ocn.z));
+  const float h = aa * (1.f - dot3_fma(l, l));
+  const float cc = dot3_fma(ocn, ocn) - 1.f;
+  const float qq = fmaf(sgnf(bp), sqrtf(fmaxf(h, 0.f)), bp);
+  const float r0 = cc / qq, r1 = qq / aa;
+  const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
+  const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
+  // entry and exit go in together when the entry lies after the cursor (and within tmax); otherwise the exit
+  // alone when it lies after the cursor and within tmax
+  const bool hit = h >= 0.f && !(t1 < S.cur_t);
+  const bool with_entry = t0 >= S.cur_t && ve > S.cursor;
+  const bool ins_e = hit && with_entry && t0 <= A.tmax;
+  const bool ins_x = hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && t1 <= A.tmax));
+  const uint64_t e = ins_e ? ve : kNoEvent, x = ins_x ? vx : kNoEvent;
+  if (e < bk[K - 1] || x < bk[K - 1]) {
+    kbuf_insert_pred<K>(bk, e);
+    kbuf_insert_pred<K>(bk, x);
+  }
+}
+
+// Integrates the gathered events in order (shaders.slang:210-226); returns true when the buffer held fewer than
+// K events, i.e. the ray has no event left. The head of the buffer is consumed and the rest shifted down, so the
+// loop body exists once.
+template <int K>
+__device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, uint64_t (&bk)[K]) {
+  for (int k = 0; k < K; ++k) {
+    const uint64_t head = bk[0];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
+    const float tk = __uint_as_float((uint32_t)(head >> 32));
+    if (tk > kEndT) return true;
+    const uint32_t key = (uint32_t)head;
+    const uint32_t s = key >> 1;
+    const bool entry = (key & 1u) == 0u;
+    const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
+    const float4 Dr = A.rec[4 * (int64_t)s + 3];
+    const uint32_t orig = s;  // records are in original primitive order
+    const f3 mu = ld3(A.means + 3 * (int64_t)orig);
+    const double t = event_depth_fp64(RA, RB, RC, mu, S.odx, S.ody, S.odz, S.ddx, S.ddy, S.ddz, entry);
+    const float hgt = entry ? Dr.x : -Dr.x;
+    // update (spline-machine.slang:103-138)
+    const float dt = fmaxf((float)(t - S.t_state), 0.f);
+    const float Df = (float)S.D0;
+    const float area = fmaxf(Df * dt, 0.f);
+    const float rn0 = safe_div((float)S.D1, Df), rn1 = safe_div((float)S.D2, Df), rn2 = safe_div((float)S.D3, Df);
+    const float e2 = expf(-(float)S.logT);
+    const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
+    S.logT = fmax((double)area + S.logT, 0.0);
+    S.C0 += w * rn0;
+    S.C1 += w * rn1;
+    S.C2 += w * rn2;
+    const float m = ladder((float)((t + S.t_state) * (0.5 * kPreMulti)));
+    S.dpx += 2.f * w * m * S.csx;
+    S.dpy += 2.f * w * S.csy;
+    S.csx += w;
+    S.csy += w * m;
+    S.D0 += (double)hgt;
+    S.D1 += (double)(hgt * Dr.y);
+    S.D2 += (double)(hgt * Dr.z);
+    S.D3 += (double)(hgt * Dr.w);
+    S.t_state = t;
+    S.cur_t = tk;
+    S.cursor = head;
+    if (A.tri) A.tri[S.r + (int64_t)S.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
+    if (A.touch) atomicAdd(A.touch + orig, 1);
+    ++S.iter;
+    if (!ray_alive(A, S)) return false;
+  }
+  return false;
+}
+
+// extract_color (spline-machine.slang:146-153) + distortion assembly (fast_ellipsoid_splinetracer.py:81-85)
+__device__ __forceinline__ void ray_write(const FwdArgs& A, const RayState& S) {
+  const int64_t r = S.r;
+  float* oc = A.color + 5 * r;
+  oc[0] = S.C0;
+  oc[1] = S.C1;
+  oc[2] = S.C2;
+  oc[3] = (float)(-expm1(-S.logT));
+  oc[4] = S.dpx - S.dpy;
+  float4* os = reinterpret_cast<float4*>(A.states + 16 * r);
+  os[0] = make_float4(S.dpx, S.dpy, S.csx, S.csy);
+  os[1] = make_float4(0.f, 0.f, 0.f, (float)S.t_state);
+  os[2] = make_float4((float)S.D0, (float)S.D1, (float)S.D2, (float)S.D3);
+  os[3] = make_float4((float)S.logT, S.C0, S.C1, S.C2);
+  A.iters[r] = S.iter;
+  double* s64 = A.st64 + 6 * r;
+  s64[0] = S.D0;
+  s64[1] = S.D1;
+  s64[2] = S.D2;
+  s64[3] = S.D3;
+  s64[4] = S.logT;
+  s64[5] = S.t_state;
+}
+
+// Per-ray traversal: one lane walks the tree for its own ray with a short LDS stack (global spill beyond
+// kLdsDepth entries). Used for rays in no particular order, where the rays of a wave share few nodes.
 template <int K>
 __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   __shared__ int32_t stk[kLdsDepth * kFwdBlock];
   const int lane = threadIdx.x;
   const int64_t slot = (int64_t)blockIdx.x * kFwdBlock + lane;
   if (slot >= A.nr) return;
-  const int64_t r = A.order[slot];
-
-  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
-  const f3 d = l2n(rdraw);
-  const f3 o = ro + A.tmin * rdraw;  // shaders.slang:178-179: origin shifted along the raw direction
-  const f3 inv = mk3(fabsf(d.x) > 1e-20f ? 1.f / d.x : copysignf(1e20f, d.x),
-                     fabsf(d.y) > 1e-20f ? 1.f / d.y : copysignf(1e20f, d.y),
-                     fabsf(d.z) > 1e-20f ? 1.f / d.z : copysignf(1e20f, d.z));
-  const double odx = (double)ro.x + (double)A.tmin * rdraw.x, ody = (double)ro.y + (double)A.tmin * rdraw.y,
-               odz = (double)ro.z + (double)A.tmin * rdraw.z;
-  double ddx = rdraw.x, ddy = rdraw.y, ddz = rdraw.z;
-  {
-    const double nn = 1.0 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-    ddx *= nn;
-    ddy *= nn;
-    ddz *= nn;
-  }
-
-  // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
-  double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
-  {
-    const int32_t ncand = (int32_t)A.header[8];
-    const f3 p = ro + A.tmin * d;
-    for (int32_t j = 0; j < ncand; ++j) {
-      const int64_t i = A.cand[j];
-      float4 q = reinterpret_cast<const float4*>(A.quats)[i];
-      float qn = 1.f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-      q = make_float4(q.x * qn, q.y * qn, q.z * qn, q.w * qn);
-      f3 v = qrot(p - ld3(A.means + 3 * i), q);
-      f3 s = ld3(A.scales + 3 * i);
-      f3 tr = mk3(v.x / s.x, v.y / s.y, v.z / s.z);
-      if (dot3(tr, tr) <= 1.f) {
-        float den = A.dens[i];
-        float c0, c1, c2;
-        if (A.color_mode == EVER_COLOR_SH) {
-          float4 cr = A.colr[i];
-          c0 = cr.y; c1 = cr.z; c2 = cr.w;
-        } else {  // the reference reads the features at flat index prim*3 + c (initialize_density.cu:140-142)
-          c0 = A.feat[3 * i + 0] * kShC0 + 0.5f;
-          c1 = A.feat[3 * i + 1] * kShC0 + 0.5f;
-          c2 = A.feat[3 * i + 2] * kShC0 + 0.5f;
-        }
-        D0 += den;
-        D1 += den * c0;
-        D2 += den * c1;
-        D3 += den * c2;
-      }
-    }
-  }
-  reinterpret_cast<float4*>(A.init_drgb)[r] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
-
-  float C0 = 0.f, C1 = 0.f, C2 = 0.f, csx = 0.f, csy = 0.f, dpx = 0.f, dpy = 0.f;
-  double t_state = 0.0, logT = 0.0;
-  // cursor over (t, key): initially nothing at t == 0 (the reference requires t > t_cur = 0)
-  float cur_t = 0.f;
-  uint64_t cursor = pack_event(0.f, 0xffffffffu);
-  int iter = 0;
+  RayState S;
+  ray_setup(A, A.order[slot], S);
+  const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
   int32_t* spill = A.spill + slot * kSpillDepth;
 
-  while (logT < (double)kLogCutoff && iter < A.max_iters) {
+  while (ray_alive(A, S)) {
     uint64_t bk[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
+    const float cur_t = S.cur_t;
     // ---- traversal: gather the K nearest events after the cursor ----
     int sp = 0;
     int32_t node = root;
@@ -265,149 +430,177 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         const float4* nd = A.nodes4 + 8 * (int64_t)node;
         const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cf = nd[6];
         const float t_hi = fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
-        float d[4];
-        int32_t c[4] = {__float_as_int(cf.x), __float_as_int(cf.y), __float_as_int(cf.z), __float_as_int(cf.w)};
+        const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
+                      c3 = __float_as_int(cf.w);
+        const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
         const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
         const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
         const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+        // sort keys: entry distance bits (>= 0, so ordered as unsigned) with the child slot in the low 2 bits;
+        // a miss sorts last. Child refs stay in four scalars: an array indexed by the sorted slot would be
+        // placed in scratch memory.
+        uint32_t key[4];
         int nhit = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          float tn;
+          const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
+          const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
+          const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
+          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), cur_t));
+          const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
           // unused slots (kEmptyChild) never hit, whatever their stored box
-          const bool h = slab(lxs[k], lys[k], lzs[k], hxs[k], hys[k], hzs[k], o, inv, cur_t, t_hi, tn) &&
-                         c[k] != kEmptyChild;
-          d[k] = h ? tn : INFINITY;
+          const bool h = tn <= tf && used[k];
+          key[k] = h ? ((__float_as_uint(tn) & ~3u) | (uint32_t)k) : 0xffffffffu;
           nhit += h ? 1 : 0;
         }
         if (nhit > 0) {
-          // sorting network on (d, c), ascending
-#define EVER_CSWAP(a, b)                                      \
-  if (d[b] < d[a]) {                                          \
-    const float td = d[a]; d[a] = d[b]; d[b] = td;            \
-    const int32_t tc = c[a]; c[a] = c[b]; c[b] = tc;          \
+#define EVER_CSWAP(a, b)                           \
+  {                                                \
+    const uint32_t lo_ = min(key[a], key[b]);      \
+    key[b] = max(key[a], key[b]);                  \
+    key[a] = lo_;                                  \
   }
           EVER_CSWAP(0, 1) EVER_CSWAP(2, 3) EVER_CSWAP(0, 2) EVER_CSWAP(1, 3) EVER_CSWAP(1, 2)
 #undef EVER_CSWAP
+          auto child = [&](uint32_t kk) {
+            const bool odd = (kk & 1u) != 0u;
+            const int32_t lo = odd ? c1 : c0, hi = odd ? c3 : c2;
+            return (kk & 2u) ? hi : lo;
+          };
 #pragma unroll
           for (int k = 3; k >= 1; --k) {
             if (k < nhit && sp < kLdsDepth + kSpillDepth) {  // bounded by 3 x (LBVH depth / 2)
-              if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = c[k];
-              else spill[sp - kLdsDepth] = c[k];
+              const int32_t ck = child(key[k]);
+              if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = ck;
+              else spill[sp - kLdsDepth] = ck;
               ++sp;
             }
           }
-          node = c[0];
+          node = child(key[0]);
           continue;
         }
       } else {
-        // leaf: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame
         const uint32_t s = (uint32_t)(~node);
         const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
-        const f3 ocn = mk3(RA.x * o.x + RA.y * o.y + RA.z * o.z + RA.w, RB.x * o.x + RB.y * o.y + RB.z * o.z + RB.w,
-                           RC.x * o.x + RC.y * o.y + RC.z * o.z + RC.w);
-        const f3 rdn = mk3(RA.x * d.x + RA.y * d.y + RA.z * d.z, RB.x * d.x + RB.y * d.y + RB.z * d.z,
-                           RC.x * d.x + RC.y * d.y + RC.z * d.z);
-        const float aa = dot3(rdn, rdn);
-        const float bp = -dot3(ocn, rdn);
-        const f3 l = ocn + (bp / aa) * rdn;
-        const float h = aa * (1.f - dot3(l, l));
-        if (h >= 0.f) {
-          const float cc = dot3(ocn, ocn) - 1.f;
-          const float qq = bp + sgnf(bp) * sqrtf(h);
-          const float r0 = cc / qq, r1 = qq / aa;
-          const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
-          // intersection program (shaders.slang:96-106) + any-hit insertion of entry and exit (:138-159), with
-          // the (t, key) cursor in place of t > t_cur
-          if (!(t1 < cur_t)) {
-            const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
-            if (t0 >= cur_t && ve > cursor) {
-              if (t0 <= A.tmax) {
-                kbuf_insert<K>(bk, ve);
-                kbuf_insert<K>(bk, vx);
-              }
-            } else if (vx > cursor && t1 <= A.tmax) {
-              kbuf_insert<K>(bk, vx);
-            }
-          }
-        }
+        leaf_test<K>(A, S, RA, RB, RC, s, bk);
       }
       if (sp == 0) break;
       --sp;
       node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
     }
-    // ---- integrate the gathered events in order (shaders.slang:210-226) ----
-    // The head of the buffer is consumed and the rest shifted down, so the loop body exists once.
-    bool end = false;
-    for (int k = 0; k < K; ++k) {
-      const uint64_t head = bk[0];
-#pragma unroll
-      for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
-      const float tk = __uint_as_float((uint32_t)(head >> 32));
-      if (tk > kEndT) {
-        end = true;
-        break;
-      }
-      const uint32_t key = (uint32_t)head;
-      const uint32_t s = key >> 1;
-      const bool entry = (key & 1u) == 0u;
-      const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
-      const float4 Dr = A.rec[4 * (int64_t)s + 3];
-      const uint32_t orig = s;  // records are in original primitive order
-      const f3 mu = ld3(A.means + 3 * (int64_t)orig);
-      const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
-      const float hgt = entry ? Dr.x : -Dr.x;
-      // update (spline-machine.slang:103-138)
-      const float dt = fmaxf((float)(t - t_state), 0.f);
-      const float Df = (float)D0;
-      const float area = fmaxf(Df * dt, 0.f);
-      const float rn0 = safe_div((float)D1, Df), rn1 = safe_div((float)D2, Df), rn2 = safe_div((float)D3, Df);
-      const float e2 = expf(-(float)logT);
-      const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
-      logT = fmax((double)area + logT, 0.0);
-      C0 += w * rn0;
-      C1 += w * rn1;
-      C2 += w * rn2;
-      const float m = ladder((float)((t + t_state) * (0.5 * kPreMulti)));
-      dpx += 2.f * w * m * csx;
-      dpy += 2.f * w * csy;
-      csx += w;
-      csy += w * m;
-      D0 += (double)hgt;
-      D1 += (double)(hgt * Dr.y);
-      D2 += (double)(hgt * Dr.z);
-      D3 += (double)(hgt * Dr.w);
-      t_state = t;
-      cur_t = tk;
-      cursor = head;
-      if (A.tri) A.tri[r + (int64_t)iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
-      if (A.touch) atomicAdd(A.touch + orig, 1);
-      ++iter;
-      if (!(logT < (double)kLogCutoff && iter < A.max_iters)) break;
-    }
-    if (end) break;
+    if (integrate_events<K>(A, S, bk)) break;
   }
+  ray_write(A, S);
+}
 
-  // extract_color (spline-machine.slang:146-153) + distortion assembly (fast_ellipsoid_splinetracer.py:81-85)
-  float* oc = A.color + 5 * r;
-  oc[0] = C0;
-  oc[1] = C1;
-  oc[2] = C2;
-  oc[3] = (float)(-expm1(-logT));
-  oc[4] = dpx - dpy;
-  float4* os = reinterpret_cast<float4*>(A.states + 16 * r);
-  os[0] = make_float4(dpx, dpy, csx, csy);
-  os[1] = make_float4(0.f, 0.f, 0.f, (float)t_state);
-  os[2] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
-  os[3] = make_float4((float)logT, C0, C1, C2);
-  A.iters[r] = iter;
-  double* s64 = A.st64 + 6 * r;
-  s64[0] = D0;
-  s64[1] = D1;
-  s64[2] = D2;
-  s64[3] = D3;
-  s64[4] = logT;
-  s64[5] = t_state;
+// Loads through the constant address space: with a wave-uniform address the compiler emits scalar loads.
+typedef __attribute__((address_space(4))) const float const_float;
+__device__ __forceinline__ float4 ld_uniform4(const float4* p, int64_t i) {
+  const const_float* q = (const const_float*)(p + i);
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+// Per-wave traversal: the 64 rays of a wave (neighbours in the Morton order of the ray sort) walk the tree
+// together. A node is entered when the box test of any live lane passes, so every lane sees a superset of the
+// nodes its own walk would visit; which events reach a lane's k-buffer does not depend on the visiting order.
+// Node and leaf records are read once per wave through the scalar cache, the stack is wave-uniform (one entry
+// per lane of two registers), and children are ordered by the entry distances of the first live lane.
+template <int K>
+__global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
+  const int lane = threadIdx.x;
+  const int64_t slot = (int64_t)blockIdx.x * kWaveBlock + lane;
+  if (slot >= A.nr) return;
+  RayState S;
+  ray_setup(A, A.order[slot], S);
+  const f3 inv = S.inv, oi = S.oi;
+  const int64_t nleaves = A.header[9];
+  const int32_t root = nleaves == 1 ? ~0 : 0;
+  bool done = false;
+
+  while (true) {
+    const bool alive = !done && ray_alive(A, S);
+    const uint64_t live = __ballot(alive);
+    if (live == 0) break;
+    const int leader = __builtin_ctzll(live);
+    uint64_t bk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
+    const float t_lo = alive ? S.cur_t : INFINITY;  // a finished lane never enters a box
+    int32_t stk0 = 0, stk1 = 0;                     // wave stack: entry i in lane i % 64 of stk(i / 64)
+    int sp = 0;
+    int32_t node = root;
+    int64_t steps = 0;
+    const int64_t max_steps = 2 * nleaves + 2;  // a tree visits each node at most once per traversal
+    while (++steps <= max_steps) {
+      if (node >= 0) {
+        const float4* nd = A.nodes4 + 8 * (int64_t)node;
+        const float4 lx = ld_uniform4(nd, 0), hx = ld_uniform4(nd, 1), ly = ld_uniform4(nd, 2),
+                     hy = ld_uniform4(nd, 3), lz = ld_uniform4(nd, 4), hz = ld_uniform4(nd, 5),
+                     cf = ld_uniform4(nd, 6);
+        const float t_hi = alive ? fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32))) : -INFINITY;
+        const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
+                      c3 = __float_as_int(cf.w);
+        const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
+        const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+        // wave-uniform sort keys: the first live lane's entry distance (+inf where it misses) with the slot in
+        // the low 2 bits; children no lane enters sort last
+        uint32_t key[4];
+        int nhit = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
+          const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
+          const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
+          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
+          const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
+          const bool h = tn <= tf;
+          const bool any = used[k] && __ballot(h) != 0;
+          const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)(h ? __float_as_uint(tn) : 0x7f800000u),
+                                                                  leader);
+          key[k] = any ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
+          nhit += any ? 1 : 0;
+        }
+        if (nhit > 0) {
+#define EVER_CSWAP(a, b)                           \
+  {                                                \
+    const uint32_t lo_ = min(key[a], key[b]);      \
+    key[b] = max(key[a], key[b]);                  \
+    key[a] = lo_;                                  \
+  }
+          EVER_CSWAP(0, 1) EVER_CSWAP(2, 3) EVER_CSWAP(0, 2) EVER_CSWAP(1, 3) EVER_CSWAP(1, 2)
+#undef EVER_CSWAP
+          auto child = [&](uint32_t kk) {
+            const bool odd = (kk & 1u) != 0u;
+            const int32_t lo = odd ? c1 : c0, hi = odd ? c3 : c2;
+            return (kk & 2u) ? hi : lo;
+          };
+#pragma unroll
+          for (int k = 3; k >= 1; --k) {
+            if (k < nhit) {  // depth of a 4-wide LBVH record tree <= 32, so sp < 3 * 32 + 1 <= 2 * 64
+              const int32_t ck = child(key[k]);
+              if (sp < 64) stk0 = (lane == sp) ? ck : stk0;
+              else stk1 = (lane == sp - 64) ? ck : stk1;
+              ++sp;
+            }
+          }
+          node = child(key[0]);
+          continue;
+        }
+      } else {
+        const uint32_t s = (uint32_t)(~node);
+        const float4* rc = A.rec + 4 * (int64_t)s;
+        const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
+        if (alive) leaf_test<K>(A, S, RA, RB, RC, s, bk);
+      }
+      if (sp == 0) break;
+      --sp;
+      node = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp) : __builtin_amdgcn_readlane(stk1, sp - 64);
+    }
+    if (alive && integrate_events<K>(A, S, bk)) done = true;
+  }
+  ray_write(A, S);
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
@@ -469,10 +662,21 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
     // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0
     A.max_iters = 0;
   }
-  switch (R.kbuf) {
-    case 8: hipLaunchKernelGGL(k_trace_forward<8>, grid, dim3(kFwdBlock), 0, st, A); break;
-    case 32: hipLaunchKernelGGL(k_trace_forward<32>, grid, dim3(kFwdBlock), 0, st, A); break;
-    default: hipLaunchKernelGGL(k_trace_forward<16>, grid, dim3(kFwdBlock), 0, st, A); break;
+  const bool sorted = R.num_rays >= kMinRaysToSort && P.num_prims > 0;
+  const bool wave = R.traversal == EVER_TRAVERSAL_WAVE || (R.traversal == EVER_TRAVERSAL_AUTO && sorted);
+  if (wave) {
+    dim3 wgrid(grid_for(R.num_rays, kWaveBlock));
+    switch (R.kbuf) {
+      case 8: hipLaunchKernelGGL(k_trace_forward_wave<8>, wgrid, dim3(kWaveBlock), 0, st, A); break;
+      case 32: hipLaunchKernelGGL(k_trace_forward_wave<32>, wgrid, dim3(kWaveBlock), 0, st, A); break;
+      default: hipLaunchKernelGGL(k_trace_forward_wave<16>, wgrid, dim3(kWaveBlock), 0, st, A); break;
+    }
+  } else {
+    switch (R.kbuf) {
+      case 8: hipLaunchKernelGGL(k_trace_forward<8>, grid, dim3(kFwdBlock), 0, st, A); break;
+      case 32: hipLaunchKernelGGL(k_trace_forward<32>, grid, dim3(kFwdBlock), 0, st, A); break;
+      default: hipLaunchKernelGGL(k_trace_forward<16>, grid, dim3(kFwdBlock), 0, st, A); break;
+    }
   }
   return hipGetLastError();
 }

@@ -69,6 +69,23 @@ def get_kbuf():
     return _KBUF
 
 
+# tree walk of the forward pass (EVER_TRAVERSAL_* in include/ever_amd.h); every mode gives the same results
+TRAVERSALS = {"auto": 0, "ray": 1, "wave": 2}
+_TRAVERSAL = os.environ.get("EVER_TRAVERSAL", "auto")
+
+
+def set_traversal(mode):
+    """Select the forward tree walk: "auto" (per-wave for sorted batches of >= 65536 rays), "ray" or "wave"."""
+    global _TRAVERSAL
+    if mode not in TRAVERSALS:
+        raise ValueError(f"traversal must be one of {sorted(TRAVERSALS)}")
+    _TRAVERSAL = mode
+
+
+def get_traversal():
+    return _TRAVERSAL
+
+
 def _check_float(x, name, last=None, device=None):
     # py_binding.cpp:33-61 (CHECK_CUDA, CHECK_CONTIGUOUS, CHECK_FLOAT, last dimension)
     if not isinstance(x, torch.Tensor):
@@ -145,7 +162,8 @@ class SavedForBackward:
         self.iters = iters
 
 
-def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None):
+def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None,
+                  traversal=None):
     """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors."""
     _check_float(rayo, "ray_origins", 3, scene.device)
     _check_float(rayd, "ray_directions", 3, scene.device)
@@ -167,7 +185,8 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     order = torch.empty((nr,), dtype=torch.int32, device=dev)
     fws_bytes = L.ever_forward_workspace_bytes(nr)
     fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
-    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF,
+                  TRAVERSALS[traversal or _TRAVERSAL], 0)
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
                        C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
@@ -205,7 +224,7 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
         out["dmeans2D"] = torch.empty((n, 2), dtype=torch.float32, device=dev)
     ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, 0)
     io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
                       C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
                       C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 1
+#define EVER_ABI_VERSION 2
 
 /* status codes */
 #define EVER_OK 0
@@ -58,6 +58,12 @@ typedef struct ever_prims {
 } ever_prims;
 
 /* Ray batch and trace options (Forward::trace_rays arguments, Forward.cpp:23-82). */
+/* Tree walk of the forward pass. AUTO takes WAVE when the call sorts the rays (at least 65536 rays, so that a
+   wave holds neighbouring rays) and RAY otherwise; both give the same events and results. */
+#define EVER_TRAVERSAL_AUTO 0
+#define EVER_TRAVERSAL_RAY 1
+#define EVER_TRAVERSAL_WAVE 2
+
 typedef struct ever_rays {
   const float* origins;    /* (R,3) */
   const float* directions; /* (R,3), need not be unit */
@@ -66,6 +72,8 @@ typedef struct ever_rays {
   float tmax;
   int32_t max_iters;
   int32_t kbuf; /* events gathered per traversal: 8, 16 (reference) or 32 */
+  int32_t traversal; /* EVER_TRAVERSAL_*: who walks the tree, one lane per ray or the 64 rays of a wave */
+  int32_t reserved;  /* 0 */
 } ever_rays;
 
 typedef struct ever_forward_out {

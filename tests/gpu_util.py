@@ -12,11 +12,12 @@ def dev():
     return torch.device("cuda:0")
 
 
-def gpu_forward(sc, o, d, tmin=0.0, tmax=100.0, max_iters=500, kbuf=None, **scene_kw):
+def gpu_forward(sc, o, d, tmin=0.0, tmax=100.0, max_iters=500, kbuf=None, traversal=None, **scene_kw):
     D = dev()
     t = [sc[k].to(D).contiguous() for k in KEYS]
     scene = T.Scene(*t, **scene_kw).build()
-    out = T.trace_forward(scene, o.to(D).contiguous(), d.to(D).contiguous(), tmin, tmax, max_iters, kbuf=kbuf)
+    out = T.trace_forward(scene, o.to(D).contiguous(), d.to(D).contiguous(), tmin, tmax, max_iters, kbuf=kbuf,
+                          traversal=traversal)
     return scene, out
 
 

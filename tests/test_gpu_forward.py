@@ -20,11 +20,12 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def test_config1_matches_golden_oracle():
-    print("case: config1 forward vs golden", flush=True)
+@pytest.mark.parametrize("traversal", ["ray", "wave"])
+def test_config1_matches_golden_oracle(traversal):
+    print(f"case: config1 forward vs golden, {traversal} traversal", flush=True)
     z = np.load(os.path.join(GOLD, "config1_fp64.npz"))
     sc, o, d = scenes.config1()
-    _, out = gpu_forward(sc, o, d)
+    _, out = gpu_forward(sc, o, d, traversal=traversal)
     col = _np(out["color"])
     np.testing.assert_allclose(col[:, :4], z["color"][:, :4], atol=1e-4, rtol=1e-4)
     np.testing.assert_allclose(col[:, 4], z["color"][:, 4], atol=1e-4, rtol=1e-4)
