@@ -203,12 +203,12 @@ __device__ __forceinline__ double event_depth_fp64(float4 RA, float4 RB, float4 
   return entry ? fmin(x0, x1) : fmax(x0, x1);
 }
 
-// Per-ray state of the forward pass: the ray in fp32 (traversal, leaf tests) and FP64 (event depths), the
-// integrator state and the (t, key) cursor.
+// Per-ray state of the forward pass: the ray in fp32 (traversal, leaf tests), the integrator state and the
+// (t, key) cursor. The FP64 ray of the event depths is rebuilt from the inputs for each batch of events, so it
+// occupies no registers during the traversal.
 struct RayState {
   int64_t r;
   f3 o, d, inv, oi;
-  double odx, ody, odz, ddx, ddy, ddz;
   double D0, D1, D2, D3, t_state, logT;
   float C0, C1, C2, csx, csy, dpx, dpy;
   float cur_t;
@@ -225,18 +225,6 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState&
               fabsf(S.d.y) > 1e-20f ? 1.f / S.d.y : copysignf(1e20f, S.d.y),
               fabsf(S.d.z) > 1e-20f ? 1.f / S.d.z : copysignf(1e20f, S.d.z));
   S.oi = mk3(S.o.x * S.inv.x, S.o.y * S.inv.y, S.o.z * S.inv.z);  // slab distances as fma(plane, inv, -o*inv)
-  S.odx = (double)ro.x + (double)A.tmin * rdraw.x;
-  S.ody = (double)ro.y + (double)A.tmin * rdraw.y;
-  S.odz = (double)ro.z + (double)A.tmin * rdraw.z;
-  S.ddx = rdraw.x;
-  S.ddy = rdraw.y;
-  S.ddz = rdraw.z;
-  {
-    const double nn = 1.0 / sqrt(S.ddx * S.ddx + S.ddy * S.ddy + S.ddz * S.ddz);
-    S.ddx *= nn;
-    S.ddy *= nn;
-    S.ddz *= nn;
-  }
   // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
   double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
   const int32_t ncand = (int32_t)A.header[8];
@@ -291,9 +279,13 @@ __device__ __forceinline__ bool ray_alive(const FwdArgs& A, const RayState& S) {
 // in vector or in scalar registers) produce bit-identical depths and therefore the same event order.
 __device__ __forceinline__ float dot3_fma(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 
-template <int K>
-__device__ __forceinline__ void leaf_test(const FwdArgs& A, const RayState& S, float4 RA, float4 RB, float4 RC,
-                                          uint32_t s, uint64_t (&bk)[K]) {
+// Leaf test of one primitive: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame, then the
+// intersection program (shaders.slang:96-106) and the any-hit choice of which events enter the k-buffer (:138-159),
+// with the (t, key) cursor in place of t > t_cur. Sets e (entry) and x (exit) to the events to insert, kNoEvent
+// where none. Entry and exit go in together when the entry lies after the cursor (and within tmax); otherwise the
+// exit alone when it lies after the cursor and within tmax.
+__device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S, float4 RA, float4 RB, float4 RC,
+                                            uint32_t s, uint64_t& e, uint64_t& x) {
 #pragma clang fp contract(off)
   const f3 o = S.o, d = S.d;
   const f3 ocn = mk3(fmaf(RA.z, o.z, fmaf(RA.y, o.y, fmaf(RA.x, o.x, RA.w))),
@@ -311,28 +303,31 @@ __device__ __forceinline__ void leaf_test(const FwdArgs& A, const RayState& S, f
   const float r0 = cc / qq, r1 = qq / aa;
   const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
   const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
-  // entry and exit go in together when the entry lies after the cursor (and within tmax); otherwise the exit
-  // alone when it lies after the cursor and within tmax
   const bool hit = h >= 0.f && !(t1 < S.cur_t);
   const bool with_entry = t0 >= S.cur_t && ve > S.cursor;
   const bool ins_e = hit && with_entry && t0 <= A.tmax;
   const bool ins_x = hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && t1 <= A.tmax));
-  const uint64_t e = ins_e ? ve : kNoEvent, x = ins_x ? vx : kNoEvent;
-  if (e < bk[K - 1] || x < bk[K - 1]) {
-    kbuf_insert_pred<K>(bk, e);
-    kbuf_insert_pred<K>(bk, x);
-  }
+  e = ins_e ? ve : kNoEvent;
+  x = ins_x ? vx : kNoEvent;
 }
 
-// Integrates the gathered events in order (shaders.slang:210-226); returns true when the buffer held fewer than
-// K events, i.e. the ray has no event left. The head of the buffer is consumed and the rest shifted down, so the
-// loop body exists once.
-template <int K>
-__device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, uint64_t (&bk)[K]) {
+// Integrates the gathered events head_at(0), head_at(1), ... in order (shaders.slang:210-226); returns true when
+// the buffer held fewer than K events, i.e. the ray has no event left.
+template <int K, typename Head>
+__device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Head head_at) {
+  // the ray in FP64: origin shifted by tmin along the raw direction, unit direction
+  const f3 ro = ld3(A.rayo + 3 * S.r), rdraw = ld3(A.rayd + 3 * S.r);
+  const double odx = (double)ro.x + (double)A.tmin * rdraw.x, ody = (double)ro.y + (double)A.tmin * rdraw.y,
+               odz = (double)ro.z + (double)A.tmin * rdraw.z;
+  double ddx = rdraw.x, ddy = rdraw.y, ddz = rdraw.z;
+  {
+    const double nn = 1.0 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    ddx *= nn;
+    ddy *= nn;
+    ddz *= nn;
+  }
   for (int k = 0; k < K; ++k) {
-    const uint64_t head = bk[0];
-#pragma unroll
-    for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
+    const uint64_t head = head_at(k);
     const float tk = __uint_as_float((uint32_t)(head >> 32));
     if (tk > kEndT) return true;
     const uint32_t key = (uint32_t)head;
@@ -342,7 +337,7 @@ __device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, 
     const float4 Dr = A.rec[4 * (int64_t)s + 3];
     const uint32_t orig = s;  // records are in original primitive order
     const f3 mu = ld3(A.means + 3 * (int64_t)orig);
-    const double t = event_depth_fp64(RA, RB, RC, mu, S.odx, S.ody, S.odz, S.ddx, S.ddy, S.ddz, entry);
+    const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
     const float hgt = entry ? Dr.x : -Dr.x;
     // update (spline-machine.slang:103-138)
     const float dt = fmaxf((float)(t - S.t_state), 0.f);
@@ -373,6 +368,17 @@ __device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, 
     if (!ray_alive(A, S)) return false;
   }
   return false;
+}
+
+template <int K>
+__device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, uint64_t (&bk)[K]) {
+  // the head of the buffer is consumed and the rest shifted down, so the loop body exists once
+  return integrate_seq<K>(A, S, [&](int) {
+    const uint64_t head = bk[0];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
+    return head;
+  });
 }
 
 // extract_color (spline-machine.slang:146-153) + distortion assembly (fast_ellipsoid_splinetracer.py:81-85)
@@ -482,7 +488,10 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       } else {
         const uint32_t s = (uint32_t)(~node);
         const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
-        leaf_test<K>(A, S, RA, RB, RC, s, bk);
+        uint64_t e, x;
+        leaf_events(A, S, RA, RB, RC, s, e, x);
+        kbuf_insert<K>(bk, e);
+        kbuf_insert<K>(bk, x);
       }
       if (sp == 0) break;
       --sp;
@@ -505,13 +514,25 @@ __device__ __forceinline__ float4 ld_uniform4(const float4* p, int64_t i) {
 // nodes its own walk would visit; which events reach a lane's k-buffer does not depend on the visiting order.
 // Node and leaf records are read once per wave through the scalar cache, the stack is wave-uniform (one entry
 // per lane of two registers), and children are ordered by the entry distances of the first live lane.
+// kSigns: every lane of the wave has the direction signs `sg` (bit a set: component a negative), so the entry
+// and exit planes of each axis are known per wave and the slab test needs no per-lane min/max.
+// The k-buffer of a wave lives in LDS (entry k of lane l at ev[k][l]); only its last entry, the culling bound,
+// is kept in a register. An insertion reads the lane's K entries, bubbles the event in and writes them back; it
+// runs only when some lane of the wave has an event below its bound.
 template <int K>
-__global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
-  const int lane = threadIdx.x;
-  const int64_t slot = (int64_t)blockIdx.x * kWaveBlock + lane;
-  if (slot >= A.nr) return;
-  RayState S;
-  ray_setup(A, A.order[slot], S);
+__device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
+  uint64_t b[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) b[k] = ev[k][lane];
+  kbuf_insert_pred<K>(b, v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) ev[k][lane] = b[k];
+  kmax = b[K - 1];
+}
+
+template <int K, bool kSigns>
+__device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int lane, uint32_t sg,
+                                           uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch) {
   const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
@@ -522,9 +543,9 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
     const uint64_t live = __ballot(alive);
     if (live == 0) break;
     const int leader = __builtin_ctzll(live);
-    uint64_t bk[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
+    for (int k = 0; k < K; ++k) ev[k][lane] = kNoEvent;
+    uint64_t kmax = kNoEvent;
     const float t_lo = alive ? S.cur_t : INFINITY;  // a finished lane never enters a box
     int32_t stk0 = 0, stk1 = 0;                     // wave stack: entry i in lane i % 64 of stk(i / 64)
     int sp = 0;
@@ -534,10 +555,13 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
     while (++steps <= max_steps) {
       if (node >= 0) {
         const float4* nd = A.nodes4 + 8 * (int64_t)node;
-        const float4 lx = ld_uniform4(nd, 0), hx = ld_uniform4(nd, 1), ly = ld_uniform4(nd, 2),
-                     hy = ld_uniform4(nd, 3), lz = ld_uniform4(nd, 4), hz = ld_uniform4(nd, 5),
+        // with known signs the rows are read as entry / exit planes (row 2a + 1 holds the upper planes of axis a)
+        const int ex = kSigns ? (int)(sg & 1u) : 0, ey = kSigns ? (int)((sg >> 1) & 1u) : 0,
+                  ez = kSigns ? (int)((sg >> 2) & 1u) : 0;
+        const float4 lx = ld_uniform4(nd, ex), hx = ld_uniform4(nd, 1 - ex), ly = ld_uniform4(nd, 2 + ey),
+                     hy = ld_uniform4(nd, 3 - ey), lz = ld_uniform4(nd, 4 + ez), hz = ld_uniform4(nd, 5 - ez),
                      cf = ld_uniform4(nd, 6);
-        const float t_hi = alive ? fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32))) : -INFINITY;
+        const float t_hi = alive ? fminf(A.tmax, __uint_as_float((uint32_t)(kmax >> 32))) : -INFINITY;
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
         const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
@@ -550,13 +574,21 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
         int nhit = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
-          const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
-          const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
-          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
-          const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
+          float tn, tf;
+          if (kSigns) {  // lxs.. hold the entry planes, hxs.. the exit planes
+            tn = fmaxf(fmaxf(fmaf(lxs[k], inv.x, -oi.x), fmaf(lys[k], inv.y, -oi.y)),
+                       fmaxf(fmaf(lzs[k], inv.z, -oi.z), t_lo));
+            tf = fminf(fminf(fmaf(hxs[k], inv.x, -oi.x), fmaf(hys[k], inv.y, -oi.y)),
+                       fminf(fmaf(hzs[k], inv.z, -oi.z), t_hi));
+          } else {
+            const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
+            const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
+            const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
+            tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
+            tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
+          }
           const bool h = tn <= tf;
-          const bool any = used[k] && __ballot(h) != 0;
+          const bool any = used[k] && __builtin_amdgcn_ballot_w64(h) != 0;
           const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)(h ? __float_as_uint(tn) : 0x7f800000u),
                                                                   leader);
           key[k] = any ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
@@ -583,6 +615,14 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
               if (sp < 64) stk0 = (lane == sp) ? ck : stk0;
               else stk1 = (lane == sp - 64) ? ck : stk1;
               ++sp;
+              // bring the record into L2 before it is popped: an asynchronous copy into a scratch area of LDS
+              // that nothing reads
+              {
+                const float* src = ck >= 0 ? (const float*)(A.nodes4 + 8 * (int64_t)ck) + (lane & 31)
+                                           : (const float*)(A.rec + 4 * (int64_t)(uint32_t)(~ck)) + (lane & 15);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                                 (__attribute__((address_space(3))) void*)pf_scratch, 4, 0, 0);
+              }
             }
           }
           node = child(key[0]);
@@ -592,14 +632,32 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
         const uint32_t s = (uint32_t)(~node);
         const float4* rc = A.rec + 4 * (int64_t)s;
         const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
-        if (alive) leaf_test<K>(A, S, RA, RB, RC, s, bk);
+        uint64_t e = kNoEvent, x = kNoEvent;
+        if (alive) leaf_events(A, S, RA, RB, RC, s, e, x);
+        if (__builtin_amdgcn_ballot_w64(e < kmax) != 0) kbuf_insert_lds<K>(ev, lane, e, kmax);
+        if (__builtin_amdgcn_ballot_w64(x < kmax) != 0) kbuf_insert_lds<K>(ev, lane, x, kmax);
       }
       if (sp == 0) break;
       --sp;
       node = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp) : __builtin_amdgcn_readlane(stk1, sp - 64);
     }
-    if (alive && integrate_events<K>(A, S, bk)) done = true;
+    if (alive && integrate_seq<K>(A, S, [&](int k) { return ev[k][lane]; })) done = true;
   }
+}
+
+template <int K>
+__global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
+  const int lane = threadIdx.x;
+  const int64_t slot = (int64_t)blockIdx.x * kWaveBlock + lane;
+  if (slot >= A.nr) return;
+  __shared__ uint64_t ev[K][kWaveBlock];
+  __shared__ uint32_t pf_scratch[kWaveBlock];
+  RayState S;
+  ray_setup(A, A.order[slot], S);
+  const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
+  const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
+  if (__ballot(sg != sg0) == 0) wave_trace<K, true>(A, S, lane, sg0, ev, pf_scratch);
+  else wave_trace<K, false>(A, S, lane, sg0, ev, pf_scratch);
   ray_write(A, S);
 }
 
