@@ -10,6 +10,7 @@ kernel (algorithmic bytes / its event-timed duration vs 8 TB/s HBM) and the CPU 
 restatement of the reference algorithm, OpenMP, timed on a bounded ray subsample of the same workload).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -40,9 +41,13 @@ def parse():
 
 
 def algorithmic_bytes(out, nr, n, d, max_iters):
-    """Forward and backward algorithmic bytes of one step (ray I/O, primitive records per distinct primitive
-    a ray touches, stored event ids, saved state, SH prepass). Distinct primitives per ray are bounded below
-    by max(#entry events, #exit events)."""
+    """Algorithmic HBM bytes of one launch of the forward trace kernel and of the backward trace kernel.
+
+    Per ray: origin + direction in (24 B) and colour, state, iteration count, start-inside density, FP64 state
+    and processing order out (forward 44 + 68 B per ray as counted here), 4 B per stored event id; per distinct
+    primitive a ray touches, one 56 B record read (and in the backward one 56 B gradient written). Distinct
+    primitives per ray are bounded below by max(#entry events, #exit events), so the figures are lower bounds.
+    The SH prepass (part of the scene build) is counted separately."""
     import torch
     iters = out["iters"].long()
     tri = out["tri_collection"].view(max_iters, nr)
@@ -58,10 +63,21 @@ def algorithmic_bytes(out, nr, n, d, max_iters):
         ent += (valid & face).sum(0)
         ext += (valid & ~face).sum(0)
     U = int(torch.maximum(ent, ext).sum())
-    sh_fwd = n * (12 + 12 * d + 16)
-    fwd = nr * 44 + U * P_BYTES + nr * 68 + 4 * E + sh_fwd
-    bwd = nr * (44 + 68) + 4 * E + U * (P_BYTES + G_BYTES) + n * (12 + 12 * d + 12 + 12 * d)
+    fwd = nr * 44 + U * P_BYTES + nr * 68 + 4 * E
+    bwd = nr * (44 + 68) + 4 * E + U * (P_BYTES + G_BYTES)
     return dict(fwd=fwd, bwd=bwd, events=E, distinct=U)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a trace kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the reading of a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return int(t[kernel]["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(sc, o, d, eye, sample, max_iters, nr_full):
@@ -107,6 +123,7 @@ def main():
         sys.exit(2)
     import torch
 
+    from ever_amd import _capi as C
     from ever_amd import scenes
     from ever_amd import tracer as T
     from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
@@ -132,22 +149,33 @@ def main():
     torch.cuda.synchronize()
     timer = T.StageTimer()
     T.stage_timer = timer
+    L = C.lib()
+    C.check(L.ever_profile_enable(1), "ever_profile_enable")  # HIP events around the two trace kernels
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    C.check(L.ever_profile_enable(0), "ever_profile_enable")
     T.stage_timer = None
     stages = timer.summary()
+    kernel_ms = {}
+    for name, which in (("forward", 0), ("backward", 1)):
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        C.check(L.ever_profile_read(which, ctypes.byref(tot), ctypes.byref(cnt)), "ever_profile_read")
+        kernel_ms[name] = tot.value / max(cnt.value, 1)
 
     # one untimed forward for the algorithmic byte count of this workload
     scene = T.Scene(*[x.detach() for x in params], color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
     out = T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters)
     ab = algorithmic_bytes(out, nr, a.prims, 16, a.max_iters)
-    fwd_ms, bwd_ms = stages.get("forward", 0.0), stages.get("backward", 0.0)
+    fwd_ms, bwd_ms = kernel_ms["forward"], kernel_ms["backward"]
     dom, dom_ms, dom_bytes = ("forward", fwd_ms, ab["fwd"]) if fwd_ms >= bwd_ms else ("backward", bwd_ms, ab["bwd"])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom)
+    wave = T.get_traversal() == "wave" or (T.get_traversal() == "auto" and nr >= 65536)
+    fwd_kernel = f"k_trace_forward{'_wave' if wave else ''}<{a.kbuf}>"
     rec = {
         "metric": METRIC,
         "value": round(nr * a.steps / dt / 1e6, 4),
@@ -165,11 +193,12 @@ def main():
                                "rebuilt per step, plain traversal", "prims": a.prims, "rays": nr, "kbuf": a.kbuf,
                    "max_iters": a.max_iters, "sh_degree": 3},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "kernel_ms": {k: round(v, 3) for k, v in kernel_ms.items()},
         "events_per_ray": round(ab["events"] / nr, 3),
-        "roofline": {"bound": "hbm", "kernel": f"k_trace_{dom} (+ its small prologue kernels)",
+        "roofline": {"bound": "hbm", "kernel": fwd_kernel if dom == "forward" else "k_trace_backward",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": int(dom_bytes)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes": int(dom_bytes), "launch_ms": round(dom_ms, 4)},
     }
     if not a.no_cpu and a.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(sc, o, d, eye, a.cpu_sample, a.max_iters, nr)

@@ -47,6 +47,8 @@ class BackwardIO(ctypes.Structure):
 
 # Every function the header declares, with its ctypes signature.
 SIGNATURES = {
+    "ever_profile_enable": ([ctypes.c_int], ctypes.c_int),
+    "ever_profile_read": ([ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "ever_abi_version": ([], ctypes.c_int),
     "ever_struct_sizes": ([ctypes.POINTER(_i64)], ctypes.c_int),
     "ever_last_error": ([], ctypes.c_char_p),

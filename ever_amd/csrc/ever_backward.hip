@@ -475,7 +475,9 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     A.drayd = B.drayd;
     A.dinit = dinit;
     A.order = B.ray_order;
+    profile_begin(EVER_KERNEL_BACKWARD, st);
     hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
+    profile_end(EVER_KERNEL_BACKWARD, st);
     if (n > 0)
       hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
                          at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,

@@ -2,6 +2,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "ever_internal.hpp"
 
@@ -49,9 +51,66 @@ int check_rays(const ever_rays* R) {
   return EVER_OK;
 }
 
+// kernel timing state (ever_profile_enable / ever_profile_read)
+bool g_profile_on = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_profile_events[2];
+hipEvent_t g_profile_open[2] = {nullptr, nullptr};
+
 }  // namespace
 
+namespace ever {
+
+void profile_begin(int kernel, hipStream_t st) {
+  if (!g_profile_on) return;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, st);
+  g_profile_open[kernel] = e;
+}
+
+void profile_end(int kernel, hipStream_t st) {
+  if (!g_profile_on || !g_profile_open[kernel]) return;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, st);
+  g_profile_events[kernel].emplace_back(g_profile_open[kernel], e);
+  g_profile_open[kernel] = nullptr;
+}
+
+}  // namespace ever
+
 extern "C" {
+
+int ever_profile_enable(int on) {
+  g_profile_on = on != 0;
+  return EVER_OK;
+}
+
+int ever_profile_read(int kernel, double* total_ms, int64_t* count) {
+  if (kernel != EVER_KERNEL_FORWARD && kernel != EVER_KERNEL_BACKWARD)
+    return fail(EVER_ERR_INVALID, "kernel must be EVER_KERNEL_FORWARD or EVER_KERNEL_BACKWARD");
+  if (!total_ms || !count) return fail(EVER_ERR_INVALID, "output pointers are null");
+  double sum = 0.0;
+  int64_t n = 0;
+  int rc = EVER_OK;
+  for (auto& pr : g_profile_events[kernel]) {
+    float ms = 0.f;
+    if (rc == EVER_OK) {
+      hipError_t e = hipEventSynchronize(pr.second);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+      rc = hip_status(e, "ever_profile_read");
+      sum += ms;
+      ++n;
+    }
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  g_profile_events[kernel].clear();
+  *total_ms = sum;
+  *count = n;
+  return rc;
+}
+
 
 int ever_abi_version(void) { return EVER_ABI_VERSION; }
 

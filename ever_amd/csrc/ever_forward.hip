@@ -722,6 +722,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   }
   const bool sorted = R.num_rays >= kMinRaysToSort && P.num_prims > 0;
   const bool wave = R.traversal == EVER_TRAVERSAL_WAVE || (R.traversal == EVER_TRAVERSAL_AUTO && sorted);
+  profile_begin(EVER_KERNEL_FORWARD, st);
   if (wave) {
     dim3 wgrid(grid_for(R.num_rays, kWaveBlock));
     switch (R.kbuf) {
@@ -736,6 +737,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
       default: hipLaunchKernelGGL(k_trace_forward<16>, grid, dim3(kFwdBlock), 0, st, A); break;
     }
   }
+  profile_end(EVER_KERNEL_FORWARD, st);
   return hipGetLastError();
 }
 

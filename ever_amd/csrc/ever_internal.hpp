@@ -154,4 +154,8 @@ inline ForwardLayout forward_layout(int64_t nr) {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Event pairs around the trace kernels while ever_profile_enable(1) is in effect (ever_capi.hip).
+void profile_begin(int kernel, hipStream_t st);
+void profile_end(int kernel, hipStream_t st);
+
 }  // namespace ever

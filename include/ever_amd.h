@@ -136,6 +136,15 @@ int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspa
 /* Per-primitive SH colour at normalize(mean - origin), unclamped (sh_kernel.slang:19-42). */
 int ever_eval_sh(const float* means, const float* features, const float* origin, int64_t num_prims,
                  int32_t feature_size, int32_t sh_degree, float* out_rgb, void* stream);
+/* Kernel timing for measurement tools (no counterpart in the reference). While enabled, every launch of the
+ * forward trace kernel (EVER_KERNEL_FORWARD) and of the backward trace kernel (EVER_KERNEL_BACKWARD) is bracketed
+ * by a pair of HIP events recorded on its stream. ever_profile_read waits for the recorded pairs of one kernel,
+ * returns their summed milliseconds and count, and forgets them. Not thread-safe; host-side only. */
+#define EVER_KERNEL_FORWARD 0
+#define EVER_KERNEL_BACKWARD 1
+int ever_profile_enable(int on);
+int ever_profile_read(int kernel, double* total_ms, int64_t* count);
+
 /* dL/dfeatures of ever_eval_sh (sh_kernel.slang:45-98); dfeatures is overwritten. */
 int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,
                           int64_t num_prims, int32_t feature_size, int32_t sh_degree, float* dfeatures,
