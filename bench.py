@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--max-iters", type=int, default=500)
     ap.add_argument("--cpu-sample", type=int, default=131072, help="rays traced by the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal form")
+    ap.add_argument("--mlp-epochs", type=int, default=12)
     return ap.parse_args()
 
 
@@ -113,6 +115,75 @@ def cpu_baseline(sc, o, d, eye, sample, max_iters, nr_full):
                 sample=(f"{len(idx)} of {nr_full} rays (every {stride}th pixel), FP32, CPU BVH build "
                         f"{t_build:.2f}s + SH prepass {t_sh:.2f}s counted once, per-ray fwd+bwd "
                         f"{1e3 * per_ray:.3f} ms extrapolated to the full image"))
+
+
+MLP_FLOPS_PER_RAY = 2 * (63 * 128 + 128 * 128 + 128 * 128 + 128)  # 81,920 (inference)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def proposal_form(a, params, og, dg, grad, eye, nr, plain_value):
+    """The learned-proposal form: the MLP is trained on the GPU (outside the timed region) on (ray, saturation
+    depth) pairs of the plain tracer from 64 random cameras at radius 3-5 (256x256 each, 4.2M pairs); each timed
+    step then runs MLP inference + proposal-bounded forward + backward."""
+    import torch
+
+    from ever_amd import proposal as PR
+    from ever_amd import tracer as T
+    from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
+    dev = og.device
+    t0 = time.perf_counter()
+    scene = T.Scene(*[x.detach() for x in params], color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
+    ro, rd, tg = PR.make_pairs(scene, n_cameras=64, width=256, height=256)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    net = PR.ProposalNet(dev, seed=0)
+    t0 = time.perf_counter()
+    loss = PR.fit(net, ro, rd, tg, epochs=a.mlp_epochs, batch=65536, lr=3e-3, tau=0.5)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    del ro, rd, tg
+
+    def step():
+        prop = net.proposal(og, dg)
+        out = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters, proposal=prop)
+        torch.autograd.backward(out, grad)
+        for p in params:
+            p.grad = None
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # inference alone, event-timed on the current stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        prop = net.proposal(og, dg)
+    e1.record()
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 5
+    with torch.no_grad():
+        plain = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters)
+        bounded = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters, proposal=prop)
+    ideal_depth = PR.saturation_depth(T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters, store_events=False))
+    sat = ideal_depth < PR.BACKGROUND_DEPTH
+    rel = ((prop[sat] - ideal_depth[sat]) / ideal_depth[sat]).abs()
+    value = nr * a.steps / dt / 1e6
+    achieved = MLP_FLOPS_PER_RAY * nr / (infer_ms * 1e-3) / 1e12
+    return {"value": round(value, 4), "unit": "Mrays/s", "ms_per_step": round(1e3 * dt / a.steps, 3),
+            "vs_plain": round(value / plain_value, 4),
+            "color_max_abs_diff_vs_plain": float((plain - bounded).abs().max()),
+            "proposal_rel_err_median": round(float(rel.median()), 4),
+            "proposal_short_frac": round(float((prop[sat] < ideal_depth[sat]).float().mean()), 4),
+            "mlp": {"inference_ms": round(infer_ms, 4), "flops_per_ray": MLP_FLOPS_PER_RAY,
+                    "achieved_tflops": round(achieved, 2), "peak_tflops": BF16_PEAK_TFLOPS,
+                    "mfma_util": round(achieved / BF16_PEAK_TFLOPS, 5), "dtype": "bf16 x bf16 -> f32"},
+            "training": {"pairs": 64 * 256 * 256, "pairs_s": round(t_pairs, 3), "fit_s": round(t_fit, 3),
+                         "epochs": a.mlp_epochs, "final_loss": round(loss, 5), "tau": 0.5}}
 
 
 def main():
@@ -200,6 +271,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": int(dom_bytes), "launch_ms": round(dom_ms, 4)},
     }
+    if not a.no_proposal:
+        rec["proposal"] = proposal_form(a, params, og, dg, grad, eye, nr, rec["value"])
     if not a.no_cpu and a.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(sc, o, d, eye, a.cpu_sample, a.max_iters, nr)
     print(json.dumps(rec), flush=True)

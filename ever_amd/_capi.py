@@ -29,7 +29,7 @@ class Prims(ctypes.Structure):
 class Rays(ctypes.Structure):
     _fields_ = [("origins", _vp), ("directions", _vp), ("num_rays", _i64), ("tmin", ctypes.c_float),
                 ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32), ("traversal", _i32),
-                ("reserved", _i32)]
+                ("proposal", _vp)]
 
 
 class ForwardOut(ctypes.Structure):
@@ -49,6 +49,15 @@ class BackwardIO(ctypes.Structure):
 SIGNATURES = {
     "ever_profile_enable": ([ctypes.c_int], ctypes.c_int),
     "ever_profile_read": ([ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ever_mlp_param_count": ([], _i64),
+    "ever_mlp_workspace_bytes": ([_i64], ctypes.c_size_t),
+    "ever_mlp_sync_bf16": ([_vp, _vp, _vp], ctypes.c_int),
+    "ever_mlp_infer": ([_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+                       ctypes.c_int),
+    "ever_mlp_train_step": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                             _i32, _vp, ctypes.c_size_t, _vp, _vp, _vp], ctypes.c_int),
+    "ever_gemm_bf16": ([_vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp, _vp],
+                       ctypes.c_int),
     "ever_abi_version": ([], ctypes.c_int),
     "ever_struct_sizes": ([ctypes.POINTER(_i64)], ctypes.c_int),
     "ever_last_error": ([], ctypes.c_char_p),

@@ -217,4 +217,57 @@ int ever_eval_sh_backward(const float* means, const float* features, const float
                     "ever_eval_sh_backward");
 }
 
+int64_t ever_mlp_param_count(void) { return ever::mlp_layout().count; }
+
+size_t ever_mlp_workspace_bytes(int64_t num_rays) {
+  if (num_rays < 0) return 0;
+  return (size_t)ever::mlp_work((num_rays + 63) / 64 * 64).total;
+}
+
+int ever_mlp_sync_bf16(const float* params, uint16_t* params_bf16, void* stream) {
+  if (!params || !params_bf16) return fail(EVER_ERR_INVALID, "null pointer");
+  return hip_status(ever::launch_mlp_sync_bf16(params, params_bf16, (hipStream_t)stream), "ever_mlp_sync_bf16");
+}
+
+int ever_mlp_infer(const float* params, const uint16_t* params_bf16, const float* origins, const float* directions,
+                   int64_t num_rays, float tmin, float margin, float* log_depth, float* proposal, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  if (num_rays < 0) return fail(EVER_ERR_INVALID, "num_rays must be >= 0");
+  if (num_rays == 0) return EVER_OK;
+  if (!params || !params_bf16 || !origins || !directions || !workspace) return fail(EVER_ERR_INVALID, "null pointer");
+  if (workspace_bytes < ever_mlp_workspace_bytes(num_rays))
+    return fail(EVER_ERR_WORKSPACE, "mlp workspace too small");
+  return hip_status(ever::launch_mlp_infer(params, params_bf16, origins, directions, num_rays, tmin, margin,
+                                           log_depth, proposal, workspace, (hipStream_t)stream),
+                    "ever_mlp_infer");
+}
+
+int ever_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v, const float* origins,
+                        const float* directions, const float* target_depth, int64_t num_rays, float tmin, float tau,
+                        float lr, int32_t step, void* workspace, size_t workspace_bytes, float* loss, float* grad,
+                        void* stream) {
+  if (num_rays <= 0) return fail(EVER_ERR_INVALID, "num_rays must be > 0");
+  if (!params || !params_bf16 || !origins || !directions || !target_depth || !workspace)
+    return fail(EVER_ERR_INVALID, "null pointer");
+  if (lr > 0.f && (!adam_m || !adam_v || step < 1)) return fail(EVER_ERR_INVALID, "Adam state and step >= 1 needed");
+  if (!(tau > 0.f && tau < 1.f)) return fail(EVER_ERR_INVALID, "tau must be in (0, 1)");
+  if (workspace_bytes < ever_mlp_workspace_bytes(num_rays))
+    return fail(EVER_ERR_WORKSPACE, "mlp workspace too small");
+  return hip_status(ever::launch_mlp_train_step(params, params_bf16, adam_m, adam_v, origins, directions,
+                                                target_depth, num_rays, tmin, tau, lr, step, workspace, loss, grad,
+                                                (hipStream_t)stream),
+                    "ever_mlp_train_step");
+}
+
+int ever_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                   int64_t M, int64_t N, int64_t K, const float* bias, int32_t relu, float* C, int32_t splits,
+                   float* part, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || M % 64 || N % 64 || K % 16)
+    return fail(EVER_ERR_INVALID, "M, N must be positive multiples of 64 and K of 16");
+  if (!A || !B || !C || splits < 1 || (splits > 1 && !part)) return fail(EVER_ERR_INVALID, "null pointer or splits");
+  return hip_status(ever::launch_gemm_public(A, sam, sak, B, sbk, sbn, M, N, K, bias, relu, C, splits, part,
+                                             (hipStream_t)stream),
+                    "ever_gemm_bf16");
+}
+
 }  // extern "C"

@@ -47,6 +47,7 @@ struct FwdArgs {
   double* st64;
   int32_t* spill;
   const uint32_t* order;  // processing order: thread slot -> ray index
+  const float* proposal;  // per-ray proposed termination depth, nullable
 };
 
 // kern_prefilter (initialize_density.cu:69-105): primitives whose AABB (create_aabbs.cu:77-114) lies within tmin
@@ -214,6 +215,7 @@ struct RayState {
   float cur_t;
   uint64_t cursor;
   int iter;
+  float bound;  // events beyond this depth are left for a later (plain) traversal; +inf once plain
 };
 
 __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState& S) {
@@ -266,10 +268,39 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState&
   S.cur_t = 0.f;
   S.cursor = pack_event(0.f, 0xffffffffu);
   S.iter = 0;
+  S.bound = INFINITY;
+}
+
+// the ray's proposal, if any (NaN or negative proposals are ignored)
+__device__ __forceinline__ void ray_bound(const FwdArgs& A, RayState& S) {
+  const float b = A.proposal[S.r];
+  S.bound = b >= 0.f ? b : INFINITY;
 }
 
 __device__ __forceinline__ bool ray_alive(const FwdArgs& A, const RayState& S) {
   return S.logT < (double)kLogCutoff && S.iter < A.max_iters;
+}
+
+// Depth at which the ray leaves the union of the root record's child boxes (+inf when the root is a leaf): a
+// proposal beyond it leaves no event for a later traversal.
+__device__ __forceinline__ float scene_exit(const FwdArgs& A, const RayState& S) {
+  if (A.header[9] <= 1) return INFINITY;
+  const float4* nd = A.nodes4;
+  const float4 cf = nd[6];
+  const bool used[4] = {__float_as_int(cf.x) != kEmptyChild, __float_as_int(cf.y) != kEmptyChild,
+                        __float_as_int(cf.z) != kEmptyChild, __float_as_int(cf.w) != kEmptyChild};
+  const float iv[3] = {S.inv.x, S.inv.y, S.inv.z}, oiv[3] = {S.oi.x, S.oi.y, S.oi.z};
+  float t = INFINITY;
+  for (int a = 0; a < 3; ++a) {
+    const float4 l = nd[2 * a], h = nd[2 * a + 1];
+    float lo = INFINITY, hi = -INFINITY;
+    lo = used[0] ? fminf(lo, l.x) : lo; hi = used[0] ? fmaxf(hi, h.x) : hi;
+    lo = used[1] ? fminf(lo, l.y) : lo; hi = used[1] ? fmaxf(hi, h.y) : hi;
+    lo = used[2] ? fminf(lo, l.z) : lo; hi = used[2] ? fmaxf(hi, h.z) : hi;
+    lo = used[3] ? fminf(lo, l.w) : lo; hi = used[3] ? fmaxf(hi, h.w) : hi;
+    t = fminf(t, fmaxf(fmaf(lo, iv[a], -oiv[a]), fmaf(hi, iv[a], -oiv[a])));
+  }
+  return t;
 }
 
 // Leaf test of one primitive: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame, then the
@@ -284,6 +315,7 @@ __device__ __forceinline__ float dot3_fma(f3 a, f3 b) { return fmaf(a.z, b.z, fm
 // with the (t, key) cursor in place of t > t_cur. Sets e (entry) and x (exit) to the events to insert, kNoEvent
 // where none. Entry and exit go in together when the entry lies after the cursor (and within tmax); otherwise the
 // exit alone when it lies after the cursor and within tmax.
+template <bool kBounded>
 __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S, float4 RA, float4 RB, float4 RC,
                                             uint32_t s, uint64_t& e, uint64_t& x) {
 #pragma clang fp contract(off)
@@ -307,8 +339,14 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
   const bool with_entry = t0 >= S.cur_t && ve > S.cursor;
   const bool ins_e = hit && with_entry && t0 <= A.tmax;
   const bool ins_x = hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && t1 <= A.tmax));
-  e = ins_e ? ve : kNoEvent;
-  x = ins_x ? vx : kNoEvent;
+  if (kBounded) {
+    // under a proposal only events up to it are gathered, so that every gathered event precedes every culled one
+    e = ins_e && t0 <= S.bound ? ve : kNoEvent;
+    x = ins_x && t1 <= S.bound ? vx : kNoEvent;
+  } else {
+    e = ins_e ? ve : kNoEvent;
+    x = ins_x ? vx : kNoEvent;
+  }
 }
 
 // Integrates the gathered events head_at(0), head_at(1), ... in order (shaders.slang:210-226); returns true when
@@ -407,7 +445,7 @@ __device__ __forceinline__ void ray_write(const FwdArgs& A, const RayState& S) {
 
 // Per-ray traversal: one lane walks the tree for its own ray with a short LDS stack (global spill beyond
 // kLdsDepth entries). Used for rays in no particular order, where the rays of a wave share few nodes.
-template <int K>
+template <int K, bool kBounded>
 __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   __shared__ int32_t stk[kLdsDepth * kFwdBlock];
   const int lane = threadIdx.x;
@@ -415,6 +453,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   if (slot >= A.nr) return;
   RayState S;
   ray_setup(A, A.order[slot], S);
+  if (kBounded) ray_bound(A, S);
   const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
@@ -435,7 +474,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         // 4-wide record: slab test of up to four children, nearest first
         const float4* nd = A.nodes4 + 8 * (int64_t)node;
         const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cf = nd[6];
-        const float t_hi = fminf(A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
+        const float t_hi = fminf(kBounded ? fminf(A.tmax, S.bound) : A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
         const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
@@ -489,7 +528,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         const uint32_t s = (uint32_t)(~node);
         const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
         uint64_t e, x;
-        leaf_events(A, S, RA, RB, RC, s, e, x);
+        leaf_events<kBounded>(A, S, RA, RB, RC, s, e, x);
         kbuf_insert<K>(bk, e);
         kbuf_insert<K>(bk, x);
       }
@@ -497,7 +536,12 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       --sp;
       node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
     }
-    if (integrate_events<K>(A, S, bk)) break;
+    if (integrate_events<K>(A, S, bk)) {
+      // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
+      // else continue without the proposal
+      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) break;
+      S.bound = INFINITY;
+    }
   }
   ray_write(A, S);
 }
@@ -530,7 +574,7 @@ __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int 
   kmax = b[K - 1];
 }
 
-template <int K, bool kSigns>
+template <int K, bool kSigns, bool kBounded>
 __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int lane, uint32_t sg,
                                            uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch) {
   const f3 inv = S.inv, oi = S.oi;
@@ -561,7 +605,9 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4 lx = ld_uniform4(nd, ex), hx = ld_uniform4(nd, 1 - ex), ly = ld_uniform4(nd, 2 + ey),
                      hy = ld_uniform4(nd, 3 - ey), lz = ld_uniform4(nd, 4 + ez), hz = ld_uniform4(nd, 5 - ez),
                      cf = ld_uniform4(nd, 6);
-        const float t_hi = alive ? fminf(A.tmax, __uint_as_float((uint32_t)(kmax >> 32))) : -INFINITY;
+        const float t_hi =
+            alive ? fminf(kBounded ? fminf(A.tmax, S.bound) : A.tmax, __uint_as_float((uint32_t)(kmax >> 32)))
+                  : -INFINITY;
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
         const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
@@ -633,7 +679,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4* rc = A.rec + 4 * (int64_t)s;
         const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
         uint64_t e = kNoEvent, x = kNoEvent;
-        if (alive) leaf_events(A, S, RA, RB, RC, s, e, x);
+        if (alive) leaf_events<kBounded>(A, S, RA, RB, RC, s, e, x);
         if (__builtin_amdgcn_ballot_w64(e < kmax) != 0) kbuf_insert_lds<K>(ev, lane, e, kmax);
         if (__builtin_amdgcn_ballot_w64(x < kmax) != 0) kbuf_insert_lds<K>(ev, lane, x, kmax);
       }
@@ -641,11 +687,16 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
       --sp;
       node = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp) : __builtin_amdgcn_readlane(stk1, sp - 64);
     }
-    if (alive && integrate_seq<K>(A, S, [&](int k) { return ev[k][lane]; })) done = true;
+    if (alive && integrate_seq<K>(A, S, [&](int k) { return ev[k][lane]; })) {
+      // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
+      // else continue without the proposal
+      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) done = true;
+      else S.bound = INFINITY;
+    }
   }
 }
 
-template <int K>
+template <int K, bool kBounded>
 __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
   const int lane = threadIdx.x;
   const int64_t slot = (int64_t)blockIdx.x * kWaveBlock + lane;
@@ -654,11 +705,18 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
   __shared__ uint32_t pf_scratch[kWaveBlock];
   RayState S;
   ray_setup(A, A.order[slot], S);
+  if (kBounded) ray_bound(A, S);
   const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
   const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
-  if (__ballot(sg != sg0) == 0) wave_trace<K, true>(A, S, lane, sg0, ev, pf_scratch);
-  else wave_trace<K, false>(A, S, lane, sg0, ev, pf_scratch);
+  if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded>(A, S, lane, sg0, ev, pf_scratch);
+  else wave_trace<K, false, kBounded>(A, S, lane, sg0, ev, pf_scratch);
   ray_write(A, S);
+}
+
+// launches the instantiation picked by (k-buffer size, proposal present)
+template <int kBlock, typename Pick>
+static void launch_k(int kbuf, bool bounded, dim3 grid, hipStream_t st, const FwdArgs& A, Pick pick) {
+  hipLaunchKernelGGL(pick(kbuf, bounded), grid, dim3(kBlock), 0, st, A);
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
@@ -715,6 +773,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.st64 = O.ray_state64;
   A.spill = at<int32_t>(O.workspace, FL.spill);
   A.order = v0;
+  A.proposal = R.proposal;
   dim3 grid(grid_for(R.num_rays, kFwdBlock));
   if (P.num_prims <= 0) {
     // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0
@@ -725,17 +784,21 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   profile_begin(EVER_KERNEL_FORWARD, st);
   if (wave) {
     dim3 wgrid(grid_for(R.num_rays, kWaveBlock));
-    switch (R.kbuf) {
-      case 8: hipLaunchKernelGGL(k_trace_forward_wave<8>, wgrid, dim3(kWaveBlock), 0, st, A); break;
-      case 32: hipLaunchKernelGGL(k_trace_forward_wave<32>, wgrid, dim3(kWaveBlock), 0, st, A); break;
-      default: hipLaunchKernelGGL(k_trace_forward_wave<16>, wgrid, dim3(kWaveBlock), 0, st, A); break;
-    }
+    launch_k<kWaveBlock>(R.kbuf, R.proposal != nullptr, wgrid, st, A,
+                         [](int k, bool b) -> void (*)(FwdArgs) {
+                           if (b) return k == 8 ? k_trace_forward_wave<8, true>
+                                         : k == 32 ? k_trace_forward_wave<32, true> : k_trace_forward_wave<16, true>;
+                           return k == 8 ? k_trace_forward_wave<8, false>
+                                         : k == 32 ? k_trace_forward_wave<32, false> : k_trace_forward_wave<16, false>;
+                         });
   } else {
-    switch (R.kbuf) {
-      case 8: hipLaunchKernelGGL(k_trace_forward<8>, grid, dim3(kFwdBlock), 0, st, A); break;
-      case 32: hipLaunchKernelGGL(k_trace_forward<32>, grid, dim3(kFwdBlock), 0, st, A); break;
-      default: hipLaunchKernelGGL(k_trace_forward<16>, grid, dim3(kFwdBlock), 0, st, A); break;
-    }
+    launch_k<kFwdBlock>(R.kbuf, R.proposal != nullptr, grid, st, A,
+                        [](int k, bool b) -> void (*)(FwdArgs) {
+                          if (b) return k == 8 ? k_trace_forward<8, true>
+                                        : k == 32 ? k_trace_forward<32, true> : k_trace_forward<16, true>;
+                          return k == 8 ? k_trace_forward<8, false>
+                                        : k == 32 ? k_trace_forward<32, false> : k_trace_forward<16, false>;
+                        });
   }
   profile_end(EVER_KERNEL_FORWARD, st);
   return hipGetLastError();

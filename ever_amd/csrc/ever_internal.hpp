@@ -154,6 +154,28 @@ inline ForwardLayout forward_layout(int64_t nr) {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Proposal network (ever_mlp.hip)
+constexpr int kMlpIn = 64;     // 63 encoded features + one zero column
+constexpr int kMlpHid = 128;   // hidden width
+constexpr int kMlpSplit = 32;  // row chunks of the weight-gradient products
+struct MlpLayout {
+  int64_t w1, b1, w2, b2, w3, b3, w4, b4, count;
+};
+struct MlpWork {
+  int64_t x, h1, h2, h3, dh, dz, dz2, y, dy, part, grad, loss, total;
+};
+MlpLayout mlp_layout();
+MlpWork mlp_work(int64_t rows_padded);
+hipError_t launch_mlp_infer(const float* p, const uint16_t* pb, const float* rayo, const float* rayd, int64_t nr,
+                            float tmin, float margin, float* y, float* proposal, void* ws, hipStream_t st);
+hipError_t launch_mlp_train_step(float* p, uint16_t* pb, float* mom, float* vel, const float* rayo,
+                                 const float* rayd, const float* target, int64_t nr, float tmin, float tau, float lr,
+                                 int step, void* ws, float* loss_out, float* grad_out, hipStream_t st);
+hipError_t launch_mlp_sync_bf16(const float* p, uint16_t* pb, hipStream_t st);
+hipError_t launch_gemm_public(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                              int64_t sbn, int64_t M, int64_t N, int64_t K, const float* bias, int relu, float* C,
+                              int splits, float* part, hipStream_t st);
+
 // Event pairs around the trace kernels while ever_profile_enable(1) is in effect (ever_capi.hip).
 void profile_begin(int kernel, hipStream_t st);
 void profile_end(int kernel, hipStream_t st);

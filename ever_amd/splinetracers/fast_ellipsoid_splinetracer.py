@@ -25,7 +25,7 @@ from .. import tracer as T
 
 
 def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax, max_prim_size, wcts, max_iters,
-                 return_extras, color_mode, sh_origin, sh_degree, activation):
+                 return_extras, color_mode, sh_origin, sh_degree, activation, proposal=None):
     ctx.device = rayo.device
     if mean.device != ctx.device:
         raise RuntimeError("mean must be on the same device as the rays")
@@ -36,7 +36,8 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     color = color.contiguous()
     scene = T.Scene(mean, scale, quat, density, color, color_mode=color_mode, sh_origin=sh_origin,
                     sh_degree=sh_degree, activation=activation).build()
-    out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=return_extras)
+    out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=return_extras,
+                          proposal=proposal)
     ctx.scene = scene
     ctx.max_iters = max_iters
     ctx.max_prim_size = max_prim_size
@@ -100,14 +101,15 @@ class SplineTracerSH(Function):
 
     @staticmethod
     def forward(ctx: Any, mean, scale, quat, density, sh_features, rayo, rayd, tmin, tmax, max_prim_size, mean2D,
-                wcts, max_iters, return_extras, sh_origin, sh_degree, activation):
+                wcts, max_iters, return_extras, sh_origin, sh_degree, activation, proposal=None):
         return _run_forward(ctx, mean, scale, quat, density, sh_features, rayo, rayd, tmin, tmax, max_prim_size,
-                            wcts, max_iters, return_extras, C.EVER_COLOR_SH, sh_origin, sh_degree, activation)
+                            wcts, max_iters, return_extras, C.EVER_COLOR_SH, sh_origin, sh_degree, activation,
+                            proposal)
 
     @staticmethod
     def backward(ctx, grad_output: torch.Tensor, *unused):
         dm, ds, dq, dd, df, dro, drd, dm2 = _run_backward(ctx, grad_output)
-        return (dm, ds, dq, dd, df, dro, drd, None, None, None, dm2, None, None, None, None, None, None)
+        return (dm, ds, dq, dd, df, dro, drd, None, None, None, dm2, None, None, None, None, None, None, None)
 
 
 def trace_rays(
@@ -136,18 +138,20 @@ trace_rays.uses_density = True
 
 def trace_rays_sh(mean, scale, quat, density, sh_features, rayo, rayd, sh_origin, sh_degree, tmin=0.0, tmax=1000,
                   max_prim_size=3, dL_dmeans2D=None, wcts=None, max_iters=500, return_extras=False,
-                  activation="softplus"):
+                  activation="softplus", proposal=None):
     """trace_rays with colour = [softplus_10](SH(sh_features, normalize(mean - sh_origin))).
 
     Equals the reference caller's eval_sh -> softplus(beta=10) -> RGB2SH -> trace_rays
     (integration_utils/gaussian_renderer/ever.py:178-204); gradients flow to sh_features only through the
-    colour (no gradient to the means through the SH direction, as eval_sh.py:68).
+    colour (no gradient to the means through the SH direction, as eval_sh.py:68). `proposal` (optional, (R,)
+    float32) bounds each ray's first traversal (ever_amd.proposal); the result is the same with or without it.
     """
     if isinstance(sh_origin, torch.Tensor):
         sh_origin = [float(v) for v in sh_origin.detach().reshape(3).cpu()]
     act = {None: 0, "none": 0, "softplus": 1}[activation]
     return SplineTracerSH.apply(mean, scale, quat, density, sh_features, rayo, rayd, tmin, tmax, max_prim_size,
-                                dL_dmeans2D, wcts, max_iters, return_extras, tuple(sh_origin), int(sh_degree), act)
+                                dL_dmeans2D, wcts, max_iters, return_extras, tuple(sh_origin), int(sh_degree), act,
+                                proposal)
 
 
 trace_rays_sh.uses_density = True

@@ -163,7 +163,7 @@ class SavedForBackward:
 
 
 def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None,
-                  traversal=None):
+                  traversal=None, proposal=None):
     """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors."""
     _check_float(rayo, "ray_origins", 3, scene.device)
     _check_float(rayd, "ray_directions", 3, scene.device)
@@ -185,8 +185,12 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     order = torch.empty((nr,), dtype=torch.int32, device=dev)
     fws_bytes = L.ever_forward_workspace_bytes(nr)
     fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
+    if proposal is not None:
+        _check_float(proposal, "proposal", None, dev)
+        if proposal.numel() != nr:
+            raise RuntimeError("proposal must hold one depth per ray")
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF,
-                  TRAVERSALS[traversal or _TRAVERSAL], 0)
+                  TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal))
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
                        C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
@@ -224,7 +228,7 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
         out["dmeans2D"] = torch.empty((n, 2), dtype=torch.float32, device=dev)
     ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, 0)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, None)
     io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
                       C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
                       C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 2
+#define EVER_ABI_VERSION 3
 
 /* status codes */
 #define EVER_OK 0
@@ -73,7 +73,12 @@ typedef struct ever_rays {
   int32_t max_iters;
   int32_t kbuf; /* events gathered per traversal: 8, 16 (reference) or 32 */
   int32_t traversal; /* EVER_TRAVERSAL_*: who walks the tree, one lane per ray or the 64 rays of a wave */
-  int32_t reserved;  /* 0 */
+  /* (R) proposed termination depth per ray, nullable. Depths are measured along the unit direction from
+   * origin + tmin * direction. A ray first gathers only events up to its proposal (nodes beyond it are not
+   * entered); if it has not terminated by then it continues with the plain walk, so results equal the plain
+   * trace's (an exit beyond tmax whose entry lies before the proposal is the one case the reference's own
+   * chunking rule treats differently). Not in the reference: the learned-proposal form. */
+  const float* proposal;
 } ever_rays;
 
 typedef struct ever_forward_out {
@@ -144,6 +149,33 @@ int ever_eval_sh(const float* means, const float* features, const float* origin,
 #define EVER_KERNEL_BACKWARD 1
 int ever_profile_enable(int on);
 int ever_profile_read(int kernel, double* total_ms, int64_t* count);
+
+/* Termination-depth proposal network (no counterpart in the reference: the learned-proposal form). A 63-dim
+ * encoding of the ray (origin + tmin * direction, unit direction) -> 3 x 128 ReLU -> 1 output y = log(depth),
+ * BF16 weights and activations on MFMA with FP32 accumulation. params: FP32 master block of
+ * ever_mlp_param_count() floats (W1 [64][128], b1, W2 [128][128], b2, W3 [128][128], b3, w4 [128], b4);
+ * params_bf16: its bf16 mirror (same offsets), refreshed by ever_mlp_sync_bf16 and by every training step.
+ * Workspaces: ever_mlp_workspace_bytes(rows). */
+int64_t ever_mlp_param_count(void);
+size_t ever_mlp_workspace_bytes(int64_t num_rays);
+int ever_mlp_sync_bf16(const float* params, uint16_t* params_bf16, void* stream);
+/* log_depth (R) and/or proposal (R) = exp(log_depth) * margin; either may be null. */
+int ever_mlp_infer(const float* params, const uint16_t* params_bf16, const float* origins, const float* directions,
+                   int64_t num_rays, float tmin, float margin, float* log_depth, float* proposal, void* workspace,
+                   size_t workspace_bytes, void* stream);
+/* One Adam step (lr > 0; lr == 0 computes the gradient only) on the pinball loss (quantile tau) of y against
+ * log(target_depth); rays with target_depth <= 0 are ignored. loss (1 float, device) and grad (param_count
+ * floats, device) are optional outputs. step counts from 1 (bias correction). */
+int ever_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v, const float* origins,
+                        const float* directions, const float* target_depth, int64_t num_rays, float tmin, float tau,
+                        float lr, int32_t step, void* workspace, size_t workspace_bytes, float* loss, float* grad,
+                        void* stream);
+/* The MFMA product the network is built on, exposed for its tests: C[M][N] (FP32, row-major) =
+ * sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn] (bf16) + bias[n], optionally ReLU. M, N multiples of 64, K of 16;
+ * splits > 1 splits K over blocks through part (splits * M * N floats). */
+int ever_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                   int64_t M, int64_t N, int64_t K, const float* bias, int32_t relu, float* C, int32_t splits,
+                   float* part, void* stream);
 
 /* dL/dfeatures of ever_eval_sh (sh_kernel.slang:45-98); dfeatures is overwritten. */
 int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,
