@@ -23,13 +23,14 @@ _i32 = ctypes.c_int32
 class Prims(ctypes.Structure):
     _fields_ = [("means", _vp), ("scales", _vp), ("quats", _vp), ("densities", _vp), ("features", _vp),
                 ("num_prims", _i64), ("feature_size", _i32), ("color_mode", _i32), ("sh_degree", _i32),
-                ("activation", _i32), ("sh_origin", ctypes.c_float * 3), ("reserved", ctypes.c_float)]
+                ("activation", _i32), ("sh_origin", ctypes.c_float * 3), ("num_views", _i32),
+                ("view_origins", _vp), ("view_colors", _vp)]
 
 
 class Rays(ctypes.Structure):
     _fields_ = [("origins", _vp), ("directions", _vp), ("num_rays", _i64), ("tmin", ctypes.c_float),
                 ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32), ("traversal", _i32),
-                ("proposal", _vp)]
+                ("proposal", _vp), ("view_index", _vp), ("tile", _i32), ("reserved", _i32)]
 
 
 class ForwardOut(ctypes.Structure):
@@ -42,7 +43,8 @@ class BackwardIO(ctypes.Structure):
     _fields_ = [("grad_out", _vp), ("iters", _vp), ("tri_collection", _vp), ("states", _vp), ("ray_state64", _vp),
                 ("ray_order", _vp), ("wcts", _vp), ("num_wcts", _i64), ("dmeans", _vp), ("dscales", _vp), ("dquats", _vp),
                 ("ddensities", _vp), ("dfeatures", _vp), ("drayo", _vp), ("drayd", _vp), ("dmeans2D", _vp),
-                ("dinitial_drgb", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t)]
+                ("dinitial_drgb", _vp), ("workspace", _vp), ("workspace_bytes", ctypes.c_size_t),
+                ("view_color_grads", _vp)]
 
 
 # Every function the header declares, with its ctypes signature.

@@ -42,13 +42,30 @@ struct BwdArgs {
   float* drayd;
   float* dinit;      // (R,4) scratch or caller buffer
   const int32_t* order;  // processing order (nullable: identity)
+  // several views: per-view (density, colour) records, each ray's view, per-view colour gradients (V,N,4)
+  const float4* vcol;
+  const int32_t* view_index;
+  float* vgrad;
+  int64_t n;
 };
+
+// Accumulator key of a primitive seen from a view: the primitive itself, or view * N + primitive with several
+// views (whose colour gradients are kept per view). Fields 0..10 (geometry, density) always go to the
+// primitive's accumulator, fields 11..13 (colour) to the view's when there are several.
+__device__ __forceinline__ void add_global(const BwdArgs& A, uint32_t key, int j, float v) {
+  if (A.vgrad) {
+    if (j >= 11) atomic_addf(A.vgrad + 4 * (int64_t)key + (j - 11), v);
+    else atomic_addf(A.acc + 16 * (int64_t)(key % (uint32_t)A.n) + j, v);
+  } else {
+    atomic_addf(A.acc + 16 * (int64_t)key + j, v);
+  }
+}
 
 // Accumulate one event's per-primitive gradient into the block's LDS table (open addressing, linear probing);
 // a primitive that finds no slot within 16 probes goes straight to the global accumulator. The table is
 // flushed to the global accumulator with one atomic per field and primitive at the end of the block, which
 // turns the per-event atomics (contended: neighbouring rays hit the same primitives) into per-block ones.
-__device__ __forceinline__ void scatter_prim(uint32_t* hkeys, float (*hvals)[kHashSlots], float* acc, uint32_t p,
+__device__ __forceinline__ void scatter_prim(const BwdArgs& A, uint32_t* hkeys, float (*hvals)[kHashSlots], uint32_t p,
                                              const float (&v)[kAccFields]) {
   uint32_t h = (p * 2654435761u) >> 22;  // 10 bits
   for (int probe = 0; probe < 16; ++probe) {
@@ -64,9 +81,8 @@ __device__ __forceinline__ void scatter_prim(uint32_t* hkeys, float (*hvals)[kHa
     }
     h = (h + 1) & (kHashSlots - 1);
   }
-  float* ac = acc + 16 * (int64_t)p;
 #pragma unroll
-  for (int j = 0; j < kAccFields; ++j) atomic_addf(ac + j, v[j]);
+  for (int j = 0; j < kAccFields; ++j) add_global(A, p, j, v[j]);
 }
 
 __global__ void k_zero_f32(float* p, int64_t n) {
@@ -150,6 +166,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32
     W = A.wcts + 16 * wi;
   }
 
+  const int32_t view = A.vcol ? A.view_index[r] : 0;
   int32_t id = A.tri[r + (int64_t)(E - 1) * A.nr];
   double tk = event_t(A, org, dir, id);
   for (int k = E - 1; k >= 0; --k) {
@@ -161,7 +178,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32
     }
     const int64_t p = id >> 1;
     const int face = id & 1;
-    const float4 cr = A.colr[p];
+    const float4 cr = A.vcol ? A.vcol[(int64_t)view * A.n + p] : A.colr[p];
     const float hgt = face ? cr.x : -cr.x;
     // reconstruct the state before event k (inverse of the forward's FP64 accumulation)
     const double Dbf0 = Da0 - (double)hgt, Dbf1 = Da1 - (double)(hgt * cr.y), Dbf2 = Da2 - (double)(hgt * cr.z),
@@ -234,7 +251,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32
     {
       const float v[kAccFields] = {-rob.x, -rob.y, -rob.z, sbar.x, sbar.y, sbar.z, qb.x, qb.y, qb.z, qb.w,
                                    densb, colb0, colb1, colb2};
-      scatter_prim(hkeys, hvals, A.acc, (uint32_t)p, v);
+      scatter_prim(A, hkeys, hvals, A.vcol ? (uint32_t)(view * A.n + p) : (uint32_t)p, v);
     }
     float* ac = A.acc + 16 * p;
     if (inv) {
@@ -280,13 +297,12 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   if (slot < A.nr) backward_ray(A, A.order ? (int64_t)A.order[slot] : slot, hkeys, hvals);
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
-    const uint32_t p = hkeys[i];
-    if (p == kEmptySlot) continue;
-    float* ac = A.acc + 16 * (int64_t)p;
+    const uint32_t key = hkeys[i];
+    if (key == kEmptySlot) continue;
 #pragma unroll
     for (int j = 0; j < kAccFields; ++j) {
       const float v = hvals[j][i];
-      if (v != 0.f) atomic_addf(ac + j, v);
+      if (v != 0.f) add_global(A, key, j, v);
     }
   }
 }
@@ -298,7 +314,9 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
                                                       const uint32_t* __restrict__ header,
                                                       const float* __restrict__ rayo, const float* __restrict__ rayd,
                                                       int64_t nr, float tmin, const float* __restrict__ dinit,
-                                                      float* __restrict__ acc) {
+                                                      float* __restrict__ acc, const float4* __restrict__ vcol,
+                                                      const int32_t* __restrict__ view_index,
+                                                      float* __restrict__ vgrad, int64_t n) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
   const int32_t ncand = (int32_t)header[8];
@@ -316,12 +334,14 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
     const f3 tr = mk3(safe_div(v.x, fmaxf(s.x, 1e-8f)), safe_div(v.y, fmaxf(s.y, 1e-8f)),
                       safe_div(v.z, fmaxf(s.z, 1e-8f)));
     if (sqrtf(dot3(tr, tr)) <= 1.f) {
-      const float4 cr = colr[p];
+      const int64_t kv = vcol ? (int64_t)view_index[r] * n + p : p;
+      const float4 cr = vcol ? vcol[kv] : colr[p];
       float* ac = acc + 16 * p;
       atomic_addf(ac + 10, gv.x + gv.y * cr.y + gv.z * cr.z + gv.w * cr.w);
-      atomic_addf(ac + 11, cr.x * gv.y);
-      atomic_addf(ac + 12, cr.x * gv.z);
-      atomic_addf(ac + 13, cr.x * gv.w);
+      float* cg = vcol ? vgrad + 4 * kv : ac + 11;
+      atomic_addf(cg + 0, cr.x * gv.y);
+      atomic_addf(cg + 1, cr.x * gv.z);
+      atomic_addf(cg + 2, cr.x * gv.w);
     }
   }
 }
@@ -344,6 +364,11 @@ struct EpiArgs {
   float* dfeat;
   float* dmeans2D;
   float* grgb;  // (N,4) colour-space gradient before the SH basis (colour mode EVER_COLOR_SH)
+  // several views: view origins (V,3) and the per-view colour gradients (V,N,4), turned in place into the
+  // colour-space gradients before the SH basis
+  const float* vorig;
+  int nv;
+  float* vgrad;
 };
 
 // dL/df[p,k,c] = Y_k(dir_p) * g[p,c], one lane per output element so the (N, d, 3) writes are coalesced.
@@ -372,6 +397,41 @@ __global__ __launch_bounds__(256) void k_sh_feature_grads(const float* __restric
     }
   }
   dfeat[e] = v;
+}
+
+// Several views: dL/df[p,k,c] = sum_v Y_k(dir_{p,v}) * g_v[p,c], one lane per primitive.
+__global__ __launch_bounds__(256) void k_sh_feature_grads_views(const float* __restrict__ vgrad,
+                                                                const float* __restrict__ means,
+                                                                const float* __restrict__ vorig, int nv, int64_t n,
+                                                                int d, int deg, float* __restrict__ dfeat) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int nc = (deg + 1) * (deg + 1);
+  float acc[16][3];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.f;
+  const f3 mean = ld3(means + 3 * p);
+  for (int v = 0; v < nv; ++v) {
+    const float4 g = reinterpret_cast<const float4*>(vgrad)[(int64_t)v * n + p];
+    if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
+    const f3 dir = sh_dir(mean, ld3(vorig + 3 * v));
+    float bas[16];
+    sh_basis(deg, dir.x, dir.y, dir.z, bas);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      acc[k][0] += bas[k] * g.x;
+      acc[k][1] += bas[k] * g.y;
+      acc[k][2] += bas[k] * g.z;
+    }
+  }
+  float* out = dfeat + (int64_t)3 * d * p;
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (k < d)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) out[3 * k + ch] = k < nc ? acc[k][ch] : 0.f;
+  for (int k = 16; k < d; ++k)
+    for (int ch = 0; ch < 3; ++ch) out[3 * k + ch] = 0.f;
 }
 
 // Per primitive: clamps (fast_ellipsoid_splinetracer.py:201-211), colour -> SH features, means2D.
@@ -415,6 +475,27 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
     // bw_sh_kernel skips primitives whose colour gradient is below 1e-8 in every channel (sh_kernel.slang:167-170)
     const bool skip = fabsf(gr[0]) < 1e-8f && fabsf(gr[1]) < 1e-8f && fabsf(gr[2]) < 1e-8f;
     reinterpret_cast<float4*>(E.grgb)[p] = skip ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(gr[0], gr[1], gr[2], 0.f);
+    // several views: the same chain per view (each view is one camera's call in the reference), in place
+    for (int v = 0; E.nv > 1 && v < E.nv; ++v) {
+      float4* gv4 = reinterpret_cast<float4*>(E.vgrad) + (int64_t)v * E.n + p;
+      const float4 cg = *gv4;
+      float g[3] = {clipf(kShC0 * cg.x, -1e3f, 1e3f) / kShC0, clipf(kShC0 * cg.y, -1e3f, 1e3f) / kShC0,
+                    clipf(kShC0 * cg.z, -1e3f, 1e3f) / kShC0};
+      if (E.activation == 1) {
+        const f3 dir = sh_dir(mean, ld3(E.vorig + 3 * v));
+        float bas[16];
+        sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
+        const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
+        const float* f = E.feat + (int64_t)3 * E.d * p;
+        for (int ch = 0; ch < 3; ++ch) {
+          float x = 0.5f;
+          for (int k = 0; k < nc; ++k) x += bas[k] * f[3 * k + ch];
+          g[ch] *= softplus10_grad(x);
+        }
+      }
+      const bool sk = fabsf(g[0]) < 1e-8f && fabsf(g[1]) < 1e-8f && fabsf(g[2]) < 1e-8f;
+      *gv4 = sk ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(g[0], g[1], g[2], 0.f);
+    }
   } else {
     df[0] = g0;
     df[1] = g1;
@@ -450,6 +531,10 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
   float* dinit = B.dinitial_drgb ? B.dinitial_drgb : at<float>(B.workspace, BL.dinit);
   const int64_t nw = B.wcts ? B.num_wcts : 0;
   if (n > 0) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(16 * n, 256)), dim3(256), 0, st, acc, 16 * n);
+  const bool views = P.num_views > 1 && n > 0;
+  if (views)
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(4 * n * P.num_views, 256)), dim3(256), 0, st, B.view_color_grads,
+                       4 * n * (int64_t)P.num_views);
   if (nw > 0) hipLaunchKernelGGL(k_invert4, dim3(grid_for(nw, 64)), dim3(64), 0, st, B.wcts, nw, inv);
   if (nr > 0) {
     BwdArgs A;
@@ -475,13 +560,17 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     A.drayd = B.drayd;
     A.dinit = dinit;
     A.order = B.ray_order;
+    A.vcol = views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
+    A.view_index = views ? R.view_index : nullptr;
+    A.vgrad = views ? B.view_color_grads : nullptr;
+    A.n = n;
     profile_begin(EVER_KERNEL_BACKWARD, st);
     hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
     profile_end(EVER_KERNEL_BACKWARD, st);
     if (n > 0)
       hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
                          at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,
-                         R.directions, nr, R.tmin, dinit, acc);
+                         R.directions, nr, R.tmin, dinit, acc, A.vcol, A.view_index, A.vgrad, n);
   }
   if (n > 0) {
     EpiArgs E;
@@ -506,11 +595,19 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     E.dfeat = B.dfeatures;
     E.dmeans2D = nw > 0 ? B.dmeans2D : nullptr;
     E.grgb = at<float>(B.workspace, BL.grgb);
+    E.vorig = views ? P.view_origins : nullptr;
+    E.nv = views ? P.num_views : 0;
+    E.vgrad = views ? B.view_color_grads : nullptr;
     hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);
-    if (P.color_mode == EVER_COLOR_SH)
-      hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n * 3 * (int64_t)P.feature_size, 256)), dim3(256), 0, st,
-                         E.grgb, P.means, P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size,
-                         P.sh_degree, B.dfeatures);
+    if (P.color_mode == EVER_COLOR_SH) {
+      if (views)
+        hipLaunchKernelGGL(k_sh_feature_grads_views, dim3(grid_for(n, 256)), dim3(256), 0, st, B.view_color_grads,
+                           P.means, P.view_origins, P.num_views, n, P.feature_size, P.sh_degree, B.dfeatures);
+      else
+        hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n * 3 * (int64_t)P.feature_size, 256)), dim3(256), 0,
+                           st, E.grgb, P.means, P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size,
+                           P.sh_degree, B.dfeatures);
+    }
   }
   return hipGetLastError();
 }

@@ -50,17 +50,30 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
     float3 c;
     const float* f = P.features + (int64_t)3 * P.feature_size * i;
     if (P.color_mode == EVER_COLOR_SH) {
-      f3 dir = sh_dir(m, mk3(P.sh_origin[0], P.sh_origin[1], P.sh_origin[2]));
-      float bas[16];
-      sh_basis(P.sh_degree, dir.x, dir.y, dir.z, bas);
-      int nc = (P.sh_degree + 1) * (P.sh_degree + 1);
-      float acc[3] = {0.5f, 0.5f, 0.5f};
-      for (int k = 0; k < nc; ++k)
-        for (int ch = 0; ch < 3; ++ch) acc[ch] += bas[k] * f[3 * k + ch];
-      if (P.activation == 1)
-        for (int ch = 0; ch < 3; ++ch) acc[ch] = softplus10(acc[ch]);
-      // colour of the deg-0 feature RGB2SH(rgb) seen through SH_C0 * f + 0.5 (ever.py:178-180, shaders.slang:65-82)
-      c = make_float3(acc[0], acc[1], acc[2]);
+      auto colour_at = [&](f3 origin) {
+        const f3 dir = sh_dir(m, origin);
+        float bas[16];
+        sh_basis(P.sh_degree, dir.x, dir.y, dir.z, bas);
+        const int nc = (P.sh_degree + 1) * (P.sh_degree + 1);
+        float acc[3] = {0.5f, 0.5f, 0.5f};
+        for (int k = 0; k < nc; ++k)
+          for (int ch = 0; ch < 3; ++ch) acc[ch] += bas[k] * f[3 * k + ch];
+        if (P.activation == 1)
+          for (int ch = 0; ch < 3; ++ch) acc[ch] = softplus10(acc[ch]);
+        // colour of the deg-0 feature RGB2SH(rgb) seen through SH_C0 * f + 0.5 (ever.py:178-180,
+        // shaders.slang:65-82)
+        return make_float3(acc[0], acc[1], acc[2]);
+      };
+      if (P.num_views > 1) {
+        float4* vc = reinterpret_cast<float4*>(P.view_colors);
+        for (int v = 0; v < P.num_views; ++v) {
+          const float3 cv = colour_at(ld3(P.view_origins + 3 * v));
+          vc[(int64_t)v * P.num_prims + i] = make_float4(dens, cv.x, cv.y, cv.z);
+          if (v == 0) c = cv;
+        }
+      } else {
+        c = colour_at(mk3(P.sh_origin[0], P.sh_origin[1], P.sh_origin[2]));
+      }
     } else {
       c = make_float3(kShC0 * f[0] + 0.5f, kShC0 * f[1] + 0.5f, kShC0 * f[2] + 0.5f);
     }

@@ -37,6 +37,14 @@ int check_prims(const ever_prims* P) {
       return fail(EVER_ERR_INVALID, "feature_size is smaller than (sh_degree+1)^2");
     if (P->activation != 0 && P->activation != 1) return fail(EVER_ERR_INVALID, "unknown activation");
   }
+  if (P->num_views < 0) return fail(EVER_ERR_INVALID, "num_views must be >= 0");
+  if (P->num_views > 1) {
+    if (P->color_mode != EVER_COLOR_SH) return fail(EVER_ERR_INVALID, "several views need color_mode EVER_COLOR_SH");
+    if (!P->view_origins || (P->num_prims > 0 && !P->view_colors))
+      return fail(EVER_ERR_INVALID, "view_origins and view_colors are needed for several views");
+    if ((int64_t)P->num_views * P->num_prims >= (int64_t(1) << 32))
+      return fail(EVER_ERR_INVALID, "num_views * num_prims must be < 2^32");
+  }
   return EVER_OK;
 }
 
@@ -48,6 +56,7 @@ int check_rays(const ever_rays* R) {
   if (R->kbuf != 8 && R->kbuf != 16 && R->kbuf != 32) return fail(EVER_ERR_INVALID, "kbuf must be 8, 16 or 32");
   if (R->traversal != EVER_TRAVERSAL_AUTO && R->traversal != EVER_TRAVERSAL_RAY && R->traversal != EVER_TRAVERSAL_WAVE)
     return fail(EVER_ERR_INVALID, "traversal must be EVER_TRAVERSAL_AUTO, _RAY or _WAVE");
+  if (R->tile < 0 || R->tile > 64) return fail(EVER_ERR_INVALID, "tile must be in [0, 64]");
   return EVER_OK;
 }
 
@@ -160,6 +169,8 @@ int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspac
   if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
   if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
     return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
+  if (prims->num_views > 1 && rays->num_rays > 0 && !rays->view_index)
+    return fail(EVER_ERR_INVALID, "view_index is needed when num_views > 1");
   if (rays->num_rays > 0) {
     if (!out->color || !out->states || !out->iters || !out->initial_drgb || !out->ray_state64 || !out->ray_order)
       return fail(EVER_ERR_INVALID, "a required forward output is null");
@@ -175,6 +186,8 @@ int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspa
   if (int rc = check_prims(prims)) return rc;
   if (int rc = check_rays(rays)) return rc;
   if (!io) return fail(EVER_ERR_INVALID, "io is null");
+  if (prims->num_views > 1 && prims->num_prims > 0 && rays->num_rays > 0 && (!rays->view_index || !io->view_color_grads))
+    return fail(EVER_ERR_INVALID, "view_index and view_color_grads are needed when num_views > 1");
   if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
   if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
     return fail(EVER_ERR_WORKSPACE, "scene workspace too small");

@@ -10,6 +10,8 @@
 // and the log-transmittance are FP64 (the accumulators exactly invertible by the adjoint); 1-exp(-x) is
 // evaluated as -expm1(-x); touch counts use atomics; the re-query continues from a (t, event key) cursor
 // instead of t > t_cur, so events of equal depth straddling two k-buffer fills are processed once.
+#include <algorithm>
+
 #include "ever_common.hpp"
 #include "ever_internal.hpp"
 
@@ -48,6 +50,10 @@ struct FwdArgs {
   int32_t* spill;
   const uint32_t* order;  // processing order: thread slot -> ray index
   const float* proposal;  // per-ray proposed termination depth, nullable
+  const float4* vcol;         // (V,N) per-view (density, colour) when several views, else null
+  const int32_t* view_index;  // (R) view of each ray when several views
+  int32_t tile;               // persistent launch: 64-ray tiles fetched at a time (0: one wave per block)
+  uint32_t* tile_counter;     // persistent launch: next tile
 };
 
 // kern_prefilter (initialize_density.cu:69-105): primitives whose AABB (create_aabbs.cu:77-114) lies within tmin
@@ -218,6 +224,7 @@ struct RayState {
   float bound;  // events beyond this depth are left for a later (plain) traversal; +inf once plain
 };
 
+template <bool kViews>
 __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState& S) {
   S.r = r;
   const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
@@ -243,7 +250,7 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState&
       float den = A.dens[i];
       float c0, c1, c2;
       if (A.color_mode == EVER_COLOR_SH) {
-        float4 cr = A.colr[i];
+        const float4 cr = kViews ? A.vcol[(int64_t)A.view_index[r] * A.n + i] : A.colr[i];
         c0 = cr.y; c1 = cr.z; c2 = cr.w;
       } else {  // the reference reads the features at flat index prim*3 + c (initialize_density.cu:140-142)
         c0 = A.feat[3 * i + 0] * kShC0 + 0.5f;
@@ -351,7 +358,7 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
 
 // Integrates the gathered events head_at(0), head_at(1), ... in order (shaders.slang:210-226); returns true when
 // the buffer held fewer than K events, i.e. the ray has no event left.
-template <int K, typename Head>
+template <int K, bool kViews, typename Head>
 __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Head head_at) {
   // the ray in FP64: origin shifted by tmin along the raw direction, unit direction
   const f3 ro = ld3(A.rayo + 3 * S.r), rdraw = ld3(A.rayd + 3 * S.r);
@@ -372,7 +379,8 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
     const uint32_t s = key >> 1;
     const bool entry = (key & 1u) == 0u;
     const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
-    const float4 Dr = A.rec[4 * (int64_t)s + 3];
+    // (density, colour): the record's, or this ray's view's when several views share the batch
+    const float4 Dr = kViews ? A.vcol[(int64_t)A.view_index[S.r] * A.n + s] : A.rec[4 * (int64_t)s + 3];
     const uint32_t orig = s;  // records are in original primitive order
     const f3 mu = ld3(A.means + 3 * (int64_t)orig);
     const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
@@ -408,10 +416,10 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
   return false;
 }
 
-template <int K>
+template <int K, bool kViews>
 __device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, uint64_t (&bk)[K]) {
   // the head of the buffer is consumed and the rest shifted down, so the loop body exists once
-  return integrate_seq<K>(A, S, [&](int) {
+  return integrate_seq<K, kViews>(A, S, [&](int) {
     const uint64_t head = bk[0];
 #pragma unroll
     for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
@@ -445,14 +453,15 @@ __device__ __forceinline__ void ray_write(const FwdArgs& A, const RayState& S) {
 
 // Per-ray traversal: one lane walks the tree for its own ray with a short LDS stack (global spill beyond
 // kLdsDepth entries). Used for rays in no particular order, where the rays of a wave share few nodes.
-template <int K, bool kBounded>
+template <int K, int kFlags>  // kFlags: 1 proposal-bounded, 2 several views
 __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
+  constexpr bool kBounded = (kFlags & 1) != 0, kViews = (kFlags & 2) != 0;
   __shared__ int32_t stk[kLdsDepth * kFwdBlock];
   const int lane = threadIdx.x;
   const int64_t slot = (int64_t)blockIdx.x * kFwdBlock + lane;
   if (slot >= A.nr) return;
   RayState S;
-  ray_setup(A, A.order[slot], S);
+  ray_setup<kViews>(A, A.order[slot], S);
   if (kBounded) ray_bound(A, S);
   const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
@@ -536,7 +545,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       --sp;
       node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
     }
-    if (integrate_events<K>(A, S, bk)) {
+    if (integrate_events<K, kViews>(A, S, bk)) {
       // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
       // else continue without the proposal
       if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) break;
@@ -574,7 +583,7 @@ __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int 
   kmax = b[K - 1];
 }
 
-template <int K, bool kSigns, bool kBounded>
+template <int K, bool kSigns, bool kBounded, bool kViews>
 __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int lane, uint32_t sg,
                                            uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch) {
   const f3 inv = S.inv, oi = S.oi;
@@ -687,7 +696,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
       --sp;
       node = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp) : __builtin_amdgcn_readlane(stk1, sp - 64);
     }
-    if (alive && integrate_seq<K>(A, S, [&](int k) { return ev[k][lane]; })) {
+    if (alive && integrate_seq<K, kViews>(A, S, [&](int k) { return ev[k][lane]; })) {
       // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
       // else continue without the proposal
       if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) done = true;
@@ -696,27 +705,66 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
   }
 }
 
-template <int K, bool kBounded>
+template <int K, int kFlags>  // kFlags: 1 proposal-bounded, 2 several views, 4 persistent
 __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
+  constexpr bool kBounded = (kFlags & 1) != 0, kViews = (kFlags & 2) != 0, kPersistent = (kFlags & 4) != 0;
   const int lane = threadIdx.x;
-  const int64_t slot = (int64_t)blockIdx.x * kWaveBlock + lane;
-  if (slot >= A.nr) return;
   __shared__ uint64_t ev[K][kWaveBlock];
   __shared__ uint32_t pf_scratch[kWaveBlock];
-  RayState S;
-  ray_setup(A, A.order[slot], S);
-  if (kBounded) ray_bound(A, S);
-  const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
-  const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
-  if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded>(A, S, lane, sg0, ev, pf_scratch);
-  else wave_trace<K, false, kBounded>(A, S, lane, sg0, ev, pf_scratch);
-  ray_write(A, S);
+  // the 64 rays of processing slots [base, base + 64); lanes past the batch take no part
+  auto trace_tile = [&](int64_t base) {
+    const int64_t slot = base + lane;
+    if (slot >= A.nr) return;
+    RayState S;
+    ray_setup<kViews>(A, A.order[slot], S);
+    if (kBounded) ray_bound(A, S);
+    const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
+    const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
+    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
+    else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
+    ray_write(A, S);
+  };
+  if (!kPersistent) {
+    trace_tile((int64_t)blockIdx.x * kWaveBlock);
+    return;
+  }
+  // persistent: fetch A.tile consecutive tiles at a time until the batch is exhausted (every wave reaches the
+  // exit once the counter passes the last tile)
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(A.tile_counter, 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    const int64_t first = (int64_t)t * A.tile * kWaveBlock;
+    if (first >= A.nr) break;
+    for (int i = 0; i < A.tile; ++i) trace_tile(first + (int64_t)i * kWaveBlock);
+  }
 }
 
-// launches the instantiation picked by (k-buffer size, proposal present)
-template <int kBlock, typename Pick>
-static void launch_k(int kbuf, bool bounded, dim3 grid, hipStream_t st, const FwdArgs& A, Pick pick) {
-  hipLaunchKernelGGL(pick(kbuf, bounded), grid, dim3(kBlock), 0, st, A);
+// the instantiation for (walk, k-buffer size, flags)
+template <int K, bool kWave>
+static void (*pick_flags(int flags))(FwdArgs) {
+  if (kWave) {
+    switch (flags) {  // persistent launches (4) exist for the plain and the several-views forms
+      case 1: return k_trace_forward_wave<K, 1>;
+      case 2: return k_trace_forward_wave<K, 2>;
+      case 3: return k_trace_forward_wave<K, 3>;
+      case 4: return k_trace_forward_wave<K, 4>;
+      case 6: return k_trace_forward_wave<K, 6>;
+      default: return k_trace_forward_wave<K, 0>;
+    }
+  }
+  switch (flags) {
+    case 1: return k_trace_forward<K, 1>;
+    case 2: return k_trace_forward<K, 2>;
+    case 3: return k_trace_forward<K, 3>;
+    default: return k_trace_forward<K, 0>;
+  }
+}
+
+template <bool kWave>
+static void (*pick_kernel(int kbuf, int flags))(FwdArgs) {
+  return kbuf == 8 ? pick_flags<8, kWave>(flags) : kbuf == 32 ? pick_flags<32, kWave>(flags)
+                                                               : pick_flags<16, kWave>(flags);
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
@@ -774,6 +822,10 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.spill = at<int32_t>(O.workspace, FL.spill);
   A.order = v0;
   A.proposal = R.proposal;
+  A.vcol = P.num_views > 1 ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
+  A.view_index = R.view_index;
+  A.tile = 0;
+  A.tile_counter = nullptr;
   dim3 grid(grid_for(R.num_rays, kFwdBlock));
   if (P.num_prims <= 0) {
     // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0
@@ -782,23 +834,26 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   const bool sorted = R.num_rays >= kMinRaysToSort && P.num_prims > 0;
   const bool wave = R.traversal == EVER_TRAVERSAL_WAVE || (R.traversal == EVER_TRAVERSAL_AUTO && sorted);
   profile_begin(EVER_KERNEL_FORWARD, st);
+  int flags = (R.proposal ? 1 : 0) | (A.vcol ? 2 : 0);
   if (wave) {
     dim3 wgrid(grid_for(R.num_rays, kWaveBlock));
-    launch_k<kWaveBlock>(R.kbuf, R.proposal != nullptr, wgrid, st, A,
-                         [](int k, bool b) -> void (*)(FwdArgs) {
-                           if (b) return k == 8 ? k_trace_forward_wave<8, true>
-                                         : k == 32 ? k_trace_forward_wave<32, true> : k_trace_forward_wave<16, true>;
-                           return k == 8 ? k_trace_forward_wave<8, false>
-                                         : k == 32 ? k_trace_forward_wave<32, false> : k_trace_forward_wave<16, false>;
-                         });
+    if (R.tile > 0 && !R.proposal) {
+      // one block per wave slot of the GPU (16 per CU), no more than there are fetches
+      int dev = 0, cus = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e != hipSuccess) return e;
+      const int64_t fetches = (R.num_rays + (int64_t)R.tile * kWaveBlock - 1) / ((int64_t)R.tile * kWaveBlock);
+      wgrid = dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(fetches, 16 * (int64_t)cus)));
+      A.tile = R.tile;
+      A.tile_counter = at<uint32_t>(O.workspace, FL.counter);
+      if (!R.proposal) flags |= 4;  // the proposal-bounded form has no persistent variant
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t*>(A.tile_counter),
+                         (int64_t)1);
+    }
+    hipLaunchKernelGGL(pick_kernel<true>(R.kbuf, flags), wgrid, dim3(kWaveBlock), 0, st, A);
   } else {
-    launch_k<kFwdBlock>(R.kbuf, R.proposal != nullptr, grid, st, A,
-                        [](int k, bool b) -> void (*)(FwdArgs) {
-                          if (b) return k == 8 ? k_trace_forward<8, true>
-                                        : k == 32 ? k_trace_forward<32, true> : k_trace_forward<16, true>;
-                          return k == 8 ? k_trace_forward<8, false>
-                                        : k == 32 ? k_trace_forward<32, false> : k_trace_forward<16, false>;
-                        });
+    hipLaunchKernelGGL(pick_kernel<false>(R.kbuf, flags), grid, dim3(kFwdBlock), 0, st, A);
   }
   profile_end(EVER_KERNEL_FORWARD, st);
   return hipGetLastError();

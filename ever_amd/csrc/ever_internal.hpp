@@ -130,7 +130,7 @@ constexpr int64_t kMinRaysToSort = 65536;  // below this the rays are traced in 
 // Forward workspace: traversal-stack spill per ray, and the ray sort (30-bit Morton key of a point on each ray).
 struct ForwardLayout {
   int64_t tiles = 0;
-  size_t spill, keys0, vals0, keys1, vals1, hist, total;
+  size_t spill, keys0, vals0, keys1, vals1, hist, counter, total;
 };
 inline ForwardLayout forward_layout(int64_t nr) {
   ForwardLayout L;
@@ -148,6 +148,7 @@ inline ForwardLayout forward_layout(int64_t nr) {
   L.keys1 = take(4 * (size_t)r);
   L.vals1 = take(4 * (size_t)r);
   L.hist = take(4 * (size_t)kRadix * (L.tiles + 1));  // + digit totals
+  L.counter = take(4);                                 // next tile of a persistent launch
   L.total = off;
   return L;
 }

@@ -106,7 +106,9 @@ class Scene:
     """Primitive set + LBVH workspace (replaces Primitives + GAS, py_binding.cpp:94-149)."""
 
     def __init__(self, means, scales, quats, densities, features, color_mode=C.EVER_COLOR_DC, sh_origin=None,
-                 sh_degree=0, activation=0):
+                 sh_degree=0, activation=0, view_origins=None):
+        """view_origins: (V,3) origins of several views traced in one batch (colour mode SH); each ray then names
+        its view (trace_forward view_index). Not in the reference, where each camera is a separate call."""
         device = means.device
         n = means.shape[0]
         _check_float(means, "means", 3, device)
@@ -134,9 +136,29 @@ class Scene:
         self.sh_degree = sh_degree
         self.activation = activation
         self.sh_origin = [0.0, 0.0, 0.0] if sh_origin is None else [float(v) for v in sh_origin]
+        self.num_views = 0
+        self.view_origins = None
+        self.view_colors = None
+        if view_origins is not None:
+            # a float32 tensor already on the device is used as is (no copy: usable under graph capture, where
+            # sh_origin should then name view 0 so that nothing is read back)
+            if torch.is_tensor(view_origins) and view_origins.device == device:
+                vo = view_origins.reshape(-1, 3)
+                _check_float(vo, "view_origins", 3, device)
+            else:
+                vo = torch.as_tensor(view_origins, dtype=torch.float32).reshape(-1, 3).to(device).contiguous()
+            if color_mode != C.EVER_COLOR_SH:
+                raise RuntimeError("several views need SH colours")
+            self.num_views = vo.shape[0]
+            self.view_origins = vo
+            if sh_origin is None:
+                self.sh_origin = [float(v) for v in vo[0].cpu()]
+            if self.num_views > 1:
+                self.view_colors = torch.empty((self.num_views, n, 4), dtype=torch.float32, device=device)
         self.prims = C.Prims(C.ptr(means), C.ptr(scales), C.ptr(quats), C.ptr(densities), C.ptr(features), n,
                              self.feature_size, color_mode, sh_degree, activation,
-                             (ctypes.c_float * 3)(*self.sh_origin), 0.0)
+                             (ctypes.c_float * 3)(*self.sh_origin), self.num_views, C.ptr(self.view_origins),
+                             C.ptr(self.view_colors))
         L = C.lib()
         self.ws_bytes = L.ever_scene_workspace_bytes(n)
         self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
@@ -163,7 +185,7 @@ class SavedForBackward:
 
 
 def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None,
-                  traversal=None, proposal=None):
+                  traversal=None, proposal=None, view_index=None, tile=0):
     """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors."""
     _check_float(rayo, "ray_origins", 3, scene.device)
     _check_float(rayd, "ray_directions", 3, scene.device)
@@ -189,8 +211,11 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
         _check_float(proposal, "proposal", None, dev)
         if proposal.numel() != nr:
             raise RuntimeError("proposal must hold one depth per ray")
+    if scene.num_views > 1:
+        if view_index is None or view_index.dtype != torch.int32 or view_index.numel() != nr:
+            raise RuntimeError("view_index (int32, one per ray) is needed for a scene with several views")
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF,
-                  TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal))
+                  TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal), C.ptr(view_index), int(tile), 0)
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
                        C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
@@ -199,7 +224,7 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
                                      ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
         _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
-                ray_state64=st64, ray_order=order, rays=rays)
+                ray_state64=st64, ray_order=order, rays=rays, view_index=view_index)
 
 
 def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
@@ -228,12 +253,17 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
         out["dmeans2D"] = torch.empty((n, 2), dtype=torch.float32, device=dev)
     ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, None)
+    vi = fwd.get("view_index")
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, None,
+                  C.ptr(vi), 0, 0)
+    vgrad = None
+    if scene.num_views > 1:
+        vgrad = torch.empty((scene.num_views, n, 4), dtype=torch.float32, device=dev)
     io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
                       C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
                       C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),
                       C.ptr(out["drayd"]), C.ptr(out.get("dmeans2D")), C.ptr(out["dinitial_drgb"]), C.ptr(ws),
-                      ws_bytes)
+                      ws_bytes, C.ptr(vgrad))
     with torch.cuda.device(dev):
         tok = _t0("backward")
         C.check(L.ever_trace_backward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,

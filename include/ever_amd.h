@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 3
+#define EVER_ABI_VERSION 4
 
 /* status codes */
 #define EVER_OK 0
@@ -54,7 +54,13 @@ typedef struct ever_prims {
   int32_t sh_degree;    /* color_mode EVER_COLOR_SH only */
   int32_t activation;   /* color_mode EVER_COLOR_SH only: 0 none, 1 softplus(beta=10) */
   float sh_origin[3];   /* color_mode EVER_COLOR_SH only */
-  float reserved;
+  /* Several views in one batch (color_mode EVER_COLOR_SH only; not in the reference, where each camera is its own
+   * call): num_views > 1 colours every primitive once per view origin (view_origins, (V,3)) into view_colors
+   * ((V,N,4) caller buffer written by ever_build_scene as (density, R, G, B)), and each ray uses its view's
+   * colours (ever_rays.view_index). num_views 0 or 1: one view at sh_origin. V * N < 2^32. */
+  int32_t num_views;
+  const float* view_origins;
+  float* view_colors;
 } ever_prims;
 
 /* Ray batch and trace options (Forward::trace_rays arguments, Forward.cpp:23-82). */
@@ -79,6 +85,12 @@ typedef struct ever_rays {
    * trace's (an exit beyond tmax whose entry lies before the proposal is the one case the reference's own
    * chunking rule treats differently). Not in the reference: the learned-proposal form. */
   const float* proposal;
+  const int32_t* view_index; /* (R) view of each ray when prims.num_views > 1, else ignored */
+  /* Persistent launch of the per-wave walk (not in the reference): 0 = one wave per block over the whole grid;
+   * t > 0 = one block per wave slot of the GPU, each fetching t consecutive 64-ray tiles at a time from a counter
+   * until the batch is done. Results do not depend on it. */
+  int32_t tile;
+  int32_t reserved;
 } ever_rays;
 
 typedef struct ever_forward_out {
@@ -114,6 +126,7 @@ typedef struct ever_backward_io {
   float* dinitial_drgb; /* (R,4) adjoint of the start-inside density; nullable */
   void* workspace;      /* ever_backward_workspace_bytes(N, R, W) bytes */
   size_t workspace_bytes;
+  float* view_color_grads; /* (V,N,4) scratch when prims.num_views > 1, else ignored */
 } ever_backward_io;
 
 int ever_abi_version(void);
