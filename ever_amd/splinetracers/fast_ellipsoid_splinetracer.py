@@ -23,6 +23,23 @@ from .. import tracer as T
 # gradient clamps of the reference's backward (fast_ellipsoid_splinetracer.py:201-211) are applied in the
 # library's per-primitive epilogue and in the per-ray adjoint kernel.
 
+# Adjoint event source. "stored": the forward keeps every ray's event list (R * max_iters int32, the reference's
+# tri_collection) for the backward. "recompute": the forward keeps none; the backward traces the batch again
+# (same scene workspace, same deterministic walk, hence the same events) into a transient list, so nothing of
+# that size is held between forward and backward. Results are identical; "recompute" costs one more forward.
+_EVENT_MODE = "stored"
+
+
+def set_event_mode(mode):
+    global _EVENT_MODE
+    if mode not in ("stored", "recompute"):
+        raise ValueError("event mode must be 'stored' or 'recompute'")
+    _EVENT_MODE = mode
+
+
+def get_event_mode():
+    return _EVENT_MODE
+
 
 def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax, max_prim_size, wcts, max_iters,
                  return_extras, color_mode, sh_origin, sh_degree, activation, proposal=None):
@@ -36,8 +53,11 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     color = color.contiguous()
     scene = T.Scene(mean, scale, quat, density, color, color_mode=color_mode, sh_origin=sh_origin,
                     sh_degree=sh_degree, activation=activation).build()
-    out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=return_extras,
-                          proposal=proposal)
+    recompute = _EVENT_MODE == "recompute" and not return_extras
+    out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=not recompute,
+                          zero_tail=return_extras, proposal=proposal)
+    ctx.recompute = recompute
+    ctx.proposal = proposal
     ctx.scene = scene
     ctx.max_iters = max_iters
     ctx.max_prim_size = max_prim_size
@@ -71,6 +91,11 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
 def _run_backward(ctx, grad_output):
     (mean, scale, quat, density, features, rayo, rayd, tri_collection, wcts, initial_drgb, states, iters, st64,
      order, _ws) = ctx.saved_tensors
+    if ctx.recompute:
+        # trace again for the event lists (scene workspace kept from the forward; same walk, same events)
+        again = T.trace_forward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, store_events=True,
+                                kbuf=ctx.kbuf, proposal=ctx.proposal)
+        tri_collection = again["tri_collection"]
     fwd = dict(tri_collection=tri_collection, states=states, iters=iters, ray_state64=st64,
                initial_drgb=initial_drgb, ray_order=order)
     g = T.trace_backward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, fwd, grad_output, wcts=wcts,
