@@ -14,6 +14,8 @@ LIB_PATH = os.environ.get("EVER_AMD_LIB", os.path.join(_HERE, "libever_amd.so"))
 EVER_OK = 0
 EVER_COLOR_DC = 0
 EVER_COLOR_SH = 1
+EVER_EVENTS_BY_RAY = 0
+EVER_EVENTS_BY_SLOT = 1
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -30,7 +32,7 @@ class Prims(ctypes.Structure):
 class Rays(ctypes.Structure):
     _fields_ = [("origins", _vp), ("directions", _vp), ("num_rays", _i64), ("tmin", ctypes.c_float),
                 ("tmax", ctypes.c_float), ("max_iters", _i32), ("kbuf", _i32), ("traversal", _i32),
-                ("proposal", _vp), ("view_index", _vp), ("tile", _i32), ("reserved", _i32)]
+                ("proposal", _vp), ("view_index", _vp), ("tile", _i32), ("event_layout", _i32)]
 
 
 class ForwardOut(ctypes.Structure):

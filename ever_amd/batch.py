@@ -79,7 +79,8 @@ class BatchStep:
         scene = T.Scene(*self.params, color_mode=1, sh_origin=self.cams[0][0], sh_degree=self.sh_degree,
                         activation=self.activation, view_origins=self.eyes).build()
         f = T.trace_forward(scene, self.rayo, self.rayd, self.tmin, self.tmax, self.max_iters, kbuf=self.kbuf,
-                            traversal="wave", view_index=self.view, tile=self.tile)
+                            traversal="wave", view_index=self.view, tile=self.tile, count_touches=False,
+                            event_layout="slot")
         b = T.trace_backward(scene, self.rayo, self.rayd, self.tmin, self.tmax, self.max_iters, f, self.grad,
                              kbuf=self.kbuf)
         return {"color": f["color"], "iters": f["iters"], **b}

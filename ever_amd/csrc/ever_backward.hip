@@ -42,6 +42,7 @@ struct BwdArgs {
   float* drayd;
   float* dinit;      // (R,4) scratch or caller buffer
   const int32_t* order;  // processing order (nullable: identity)
+  int32_t events_by_slot;  // tri column = processing slot instead of ray index
   // several views: per-view (density, colour) records, each ray's view, per-view colour gradients (V,N,4)
   const float4* vcol;
   const int32_t* view_index;
@@ -128,7 +129,7 @@ __device__ __forceinline__ double event_t(const BwdArgs& A, d3 org, f3 dir, int3
   return safe_root_select(S, id & 1);
 }
 
-__device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32_t* hkeys,
+__device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_t col, uint32_t* hkeys,
                                              float (*hvals)[kHashSlots]) {
   const int E = A.iters[r];
   float4* dinit = reinterpret_cast<float4*>(A.dinit) + r;
@@ -167,13 +168,13 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, uint32
   }
 
   const int32_t view = A.vcol ? A.view_index[r] : 0;
-  int32_t id = A.tri[r + (int64_t)(E - 1) * A.nr];
+  int32_t id = A.tri[col + (int64_t)(E - 1) * A.nr];
   double tk = event_t(A, org, dir, id);
   for (int k = E - 1; k >= 0; --k) {
     int32_t id_prev = 0;
     double tprev = 0.0;
     if (k > 0) {
-      id_prev = A.tri[r + (int64_t)(k - 1) * A.nr];
+      id_prev = A.tri[col + (int64_t)(k - 1) * A.nr];
       tprev = event_t(A, org, dir, id_prev);
     }
     const int64_t p = id >> 1;
@@ -294,7 +295,10 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   }
   __syncthreads();
   const int64_t slot = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
-  if (slot < A.nr) backward_ray(A, A.order ? (int64_t)A.order[slot] : slot, hkeys, hvals);
+  if (slot < A.nr) {
+    const int64_t r = A.order ? (int64_t)A.order[slot] : slot;
+    backward_ray(A, r, A.events_by_slot ? slot : r, hkeys, hvals);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
     const uint32_t key = hkeys[i];
@@ -560,6 +564,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     A.drayd = B.drayd;
     A.dinit = dinit;
     A.order = B.ray_order;
+    A.events_by_slot = R.event_layout == EVER_EVENTS_BY_SLOT ? 1 : 0;
     A.vcol = views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
     A.view_index = views ? R.view_index : nullptr;
     A.vgrad = views ? B.view_color_grads : nullptr;

@@ -466,25 +466,44 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const float4 e = nodes2[i];
-  const int32_t ch[2] = {__float_as_int(e.x), __float_as_int(e.y)};
-  int32_t ref[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
-  Box bx[4] = {box_empty(), box_empty(), box_empty(), box_empty()};
-  int m = 0;
-  for (int c = 0; c < 2; ++c) {
-    if (ch[c] < 0) {
-      ref[m] = ~(int32_t)fprim[order[~ch[c]]];
-      bx[m] = box_load(sbox + 6 * (int64_t)(~ch[c]));
-      ++m;
-    } else {
-      const float4 ge = nodes2[ch[c]];
-      const int32_t g[2] = {__float_as_int(ge.x), __float_as_int(ge.y)};
-      for (int k = 0; k < 2; ++k) {
-        bx[m] = child_box(g[k], range, sbox, pre, suf, tab, nb);
-        ref[m] = g[k] < 0 ? ~(int32_t)fprim[order[~g[k]]] : g[k];
-        ++m;
-      }
-    }
+  // the (up to) four grandchildren in order: a leaf child stands for itself, an internal child for its two
+  // children; the slots are filled with selects (no dynamically indexed arrays, which would live in scratch)
+  const int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+  int32_t ra0, ra1 = kEmptyChild, rb0, rb1 = kEmptyChild;
+  Box ba0, ba1 = box_empty(), bb0, bb1 = box_empty();
+  if (c0 < 0) {
+    ra0 = ~(int32_t)fprim[order[~c0]];
+    ba0 = box_load(sbox + 6 * (int64_t)(~c0));
+  } else {
+    const float4 ge = nodes2[c0];
+    const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
+    ba0 = child_box(g0, range, sbox, pre, suf, tab, nb);
+    ra0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
+    ba1 = child_box(g1, range, sbox, pre, suf, tab, nb);
+    ra1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
   }
+  if (c1 < 0) {
+    rb0 = ~(int32_t)fprim[order[~c1]];
+    bb0 = box_load(sbox + 6 * (int64_t)(~c1));
+  } else {
+    const float4 ge = nodes2[c1];
+    const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
+    bb0 = child_box(g0, range, sbox, pre, suf, tab, nb);
+    rb0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
+    bb1 = child_box(g1, range, sbox, pre, suf, tab, nb);
+    rb1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
+  }
+  const bool two = c0 >= 0;  // child 0 took two slots
+  int32_t ref[4];
+  Box bx[4];
+  ref[0] = ra0;
+  bx[0] = ba0;
+  ref[1] = two ? ra1 : rb0;
+  bx[1] = two ? ba1 : bb0;
+  ref[2] = two ? rb0 : rb1;
+  bx[2] = two ? bb0 : bb1;
+  ref[3] = two ? rb1 : kEmptyChild;
+  bx[3] = two ? bb1 : box_empty();
   float4* nd = nodes4 + 8 * i;
   nd[0] = make_float4(bx[0].lx, bx[1].lx, bx[2].lx, bx[3].lx);
   nd[1] = make_float4(bx[0].hx, bx[1].hx, bx[2].hx, bx[3].hx);

@@ -57,6 +57,9 @@ int check_rays(const ever_rays* R) {
   if (R->traversal != EVER_TRAVERSAL_AUTO && R->traversal != EVER_TRAVERSAL_RAY && R->traversal != EVER_TRAVERSAL_WAVE)
     return fail(EVER_ERR_INVALID, "traversal must be EVER_TRAVERSAL_AUTO, _RAY or _WAVE");
   if (R->tile < 0 || R->tile > 64) return fail(EVER_ERR_INVALID, "tile must be in [0, 64]");
+  if (R->event_layout != EVER_EVENTS_BY_RAY && R->event_layout != EVER_EVENTS_BY_SLOT)
+    return fail(EVER_ERR_INVALID, "event_layout must be EVER_EVENTS_BY_RAY or EVER_EVENTS_BY_SLOT");
+  if (R->num_rays > INT32_MAX) return fail(EVER_ERR_INVALID, "num_rays must be < 2^31");
   return EVER_OK;
 }
 

@@ -53,6 +53,7 @@ struct FwdArgs {
   const float4* vcol;         // (V,N) per-view (density, colour) when several views, else null
   const int32_t* view_index;  // (R) view of each ray when several views
   int32_t tile;               // persistent launch: 64-ray tiles fetched at a time (0: one wave per block)
+  int32_t events_by_slot;     // tri column = processing slot instead of ray index
   uint32_t* tile_counter;     // persistent launch: next tile
 };
 
@@ -214,7 +215,8 @@ __device__ __forceinline__ double event_depth_fp64(float4 RA, float4 RB, float4 
 // (t, key) cursor. The FP64 ray of the event depths is rebuilt from the inputs for each batch of events, so it
 // occupies no registers during the traversal.
 struct RayState {
-  int64_t r;
+  int32_t r;  // ray index
+  int32_t e;  // column of the ray's events in tri: the ray index, or its processing slot (events by slot)
   f3 o, d, inv, oi;
   double D0, D1, D2, D3, t_state, logT;
   float C0, C1, C2, csx, csy, dpx, dpy;
@@ -225,8 +227,9 @@ struct RayState {
 };
 
 template <bool kViews>
-__device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, RayState& S) {
-  S.r = r;
+__device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t slot, RayState& S) {
+  S.r = (int32_t)r;
+  S.e = (int32_t)(A.events_by_slot ? slot : r);
   const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
   S.d = l2n(rdraw);
   S.o = ro + A.tmin * rdraw;  // shaders.slang:178-179: origin shifted along the raw direction
@@ -361,7 +364,7 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
 template <int K, bool kViews, typename Head>
 __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Head head_at) {
   // the ray in FP64: origin shifted by tmin along the raw direction, unit direction
-  const f3 ro = ld3(A.rayo + 3 * S.r), rdraw = ld3(A.rayd + 3 * S.r);
+  const f3 ro = ld3(A.rayo + 3 * (int64_t)S.r), rdraw = ld3(A.rayd + 3 * (int64_t)S.r);
   const double odx = (double)ro.x + (double)A.tmin * rdraw.x, ody = (double)ro.y + (double)A.tmin * rdraw.y,
                odz = (double)ro.z + (double)A.tmin * rdraw.z;
   double ddx = rdraw.x, ddy = rdraw.y, ddz = rdraw.z;
@@ -408,7 +411,7 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
     S.t_state = t;
     S.cur_t = tk;
     S.cursor = head;
-    if (A.tri) A.tri[S.r + (int64_t)S.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
+    if (A.tri) A.tri[S.e + (int64_t)S.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
     if (A.touch) atomicAdd(A.touch + orig, 1);
     ++S.iter;
     if (!ray_alive(A, S)) return false;
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   const int64_t slot = (int64_t)blockIdx.x * kFwdBlock + lane;
   if (slot >= A.nr) return;
   RayState S;
-  ray_setup<kViews>(A, A.order[slot], S);
+  ray_setup<kViews>(A, A.order[slot], slot, S);
   if (kBounded) ray_bound(A, S);
   const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
@@ -574,13 +577,17 @@ __device__ __forceinline__ float4 ld_uniform4(const float4* p, int64_t i) {
 // runs only when some lane of the wave has an event below its bound.
 template <int K>
 __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
-  uint64_t b[K];
+  if (!(v < kmax)) return;
+  // position = number of entries below v (binary search over the sorted entries; < K since v < the last)
+  int lo = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) b[k] = ev[k][lane];
-  kbuf_insert_pred<K>(b, v);
-#pragma unroll
-  for (int k = 0; k < K; ++k) ev[k][lane] = b[k];
-  kmax = b[K - 1];
+  for (int step = K / 2; step >= 1; step >>= 1)
+    if (ev[lo + step - 1][lane] < v) lo += step;
+  if (ev[lo][lane] == v) return;  // the fragments of one primitive report identical events: keep one
+  // shift the entries from the position up by one (events mostly arrive near-first, so few move)
+  for (int j = K - 1; j > lo; --j) ev[j][lane] = ev[j - 1][lane];
+  ev[lo][lane] = v;
+  kmax = ev[K - 1][lane];
 }
 
 template <int K, bool kSigns, bool kBounded, bool kViews>
@@ -716,7 +723,7 @@ __global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
     const int64_t slot = base + lane;
     if (slot >= A.nr) return;
     RayState S;
-    ray_setup<kViews>(A, A.order[slot], S);
+    ray_setup<kViews>(A, A.order[slot], slot, S);
     if (kBounded) ray_bound(A, S);
     const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
     const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
@@ -816,6 +823,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.states = O.states;
   A.iters = O.iters;
   A.tri = O.tri_collection;
+  A.events_by_slot = R.event_layout == EVER_EVENTS_BY_SLOT ? 1 : 0;
   A.touch = O.touch_count;
   A.init_drgb = O.initial_drgb;
   A.st64 = O.ray_state64;

@@ -54,8 +54,15 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     scene = T.Scene(mean, scale, quat, density, color, color_mode=color_mode, sh_origin=sh_origin,
                     sh_degree=sh_degree, activation=activation).build()
     recompute = _EVENT_MODE == "recompute" and not return_extras
+    # The forward's per-primitive touch counts reach the caller only through the extras (the reference's backward
+    # counts its own, fast_ellipsoid_splinetracer.py:137), so they are only counted when extras are asked for.
+    # Event lists the caller never sees are stored by processing slot (contiguous rows per wave); the extras'
+    # tri_collection keeps the reference's by-ray layout.
+    layout = "ray" if return_extras else "slot"
     out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=not recompute,
-                          zero_tail=return_extras, proposal=proposal)
+                          zero_tail=return_extras, proposal=proposal, count_touches=return_extras,
+                          event_layout=layout)
+    ctx.event_layout = layout
     ctx.recompute = recompute
     ctx.proposal = proposal
     ctx.scene = scene
@@ -94,10 +101,11 @@ def _run_backward(ctx, grad_output):
     if ctx.recompute:
         # trace again for the event lists (scene workspace kept from the forward; same walk, same events)
         again = T.trace_forward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, store_events=True,
-                                kbuf=ctx.kbuf, proposal=ctx.proposal)
+                                kbuf=ctx.kbuf, proposal=ctx.proposal, count_touches=False,
+                                event_layout=ctx.event_layout)
         tri_collection = again["tri_collection"]
     fwd = dict(tri_collection=tri_collection, states=states, iters=iters, ray_state64=st64,
-               initial_drgb=initial_drgb, ray_order=order)
+               initial_drgb=initial_drgb, ray_order=order, event_layout=ctx.event_layout)
     g = T.trace_backward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, fwd, grad_output, wcts=wcts,
                          kbuf=ctx.kbuf)
     return (g["dmeans"], g["dscales"], g["dquats"], g["ddensities"].reshape(density.shape), g["dfeatures"],

@@ -71,6 +71,8 @@ def get_kbuf():
 
 # tree walk of the forward pass (EVER_TRAVERSAL_* in include/ever_amd.h); every mode gives the same results
 TRAVERSALS = {"auto": 0, "ray": 1, "wave": 2}
+# event-list layouts (EVER_EVENTS_BY_* in include/ever_amd.h)
+EVENT_LAYOUTS = {"ray": C.EVER_EVENTS_BY_RAY, "slot": C.EVER_EVENTS_BY_SLOT}
 _TRAVERSAL = os.environ.get("EVER_TRAVERSAL", "auto")
 
 
@@ -185,8 +187,16 @@ class SavedForBackward:
 
 
 def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None,
-                  traversal=None, proposal=None, view_index=None, tile=0):
-    """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors."""
+                  traversal=None, proposal=None, view_index=None, tile=0, count_touches=True, event_layout="ray"):
+    """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors.
+
+    count_touches=False skips the per-primitive event counts (touch_count is then None): one global atomic per
+    event that only the caller's extras ever read. event_layout "ray" stores event k of ray r at
+    tri_collection[r + k*R] (the reference's layout); "slot" stores it at the ray's position in ray_order, so that
+    the rays traced together write, and the backward reads, contiguous rows (trace_backward follows the forward's
+    choice)."""
+    if event_layout not in EVENT_LAYOUTS:
+        raise ValueError("event_layout must be 'ray' or 'slot'")
     _check_float(rayo, "ray_origins", 3, scene.device)
     _check_float(rayd, "ray_directions", 3, scene.device)
     nr = rayo.numel() // 3
@@ -201,7 +211,7 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     iters = torch.empty((nr,), dtype=torch.int32, device=dev)
     alloc = torch.zeros if zero_tail else torch.empty
     tri = alloc((nr * max_iters,), dtype=torch.int32, device=dev) if store_events else None
-    touch = torch.empty((scene.n,), dtype=torch.int32, device=dev)
+    touch = torch.empty((scene.n,), dtype=torch.int32, device=dev) if count_touches else None
     init = torch.empty((nr, 4), dtype=torch.float32, device=dev)
     st64 = torch.empty((nr, 6), dtype=torch.float64, device=dev)
     order = torch.empty((nr,), dtype=torch.int32, device=dev)
@@ -215,7 +225,8 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
         if view_index is None or view_index.dtype != torch.int32 or view_index.numel() != nr:
             raise RuntimeError("view_index (int32, one per ray) is needed for a scene with several views")
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF,
-                  TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal), C.ptr(view_index), int(tile), 0)
+                  TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal), C.ptr(view_index), int(tile),
+                  EVENT_LAYOUTS[event_layout])
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
                        C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
     with torch.cuda.device(dev):
@@ -224,7 +235,7 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
                                      ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
         _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
-                ray_state64=st64, ray_order=order, rays=rays, view_index=view_index)
+                ray_state64=st64, ray_order=order, rays=rays, view_index=view_index, event_layout=event_layout)
 
 
 def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
@@ -255,7 +266,7 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     vi = fwd.get("view_index")
     rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, None,
-                  C.ptr(vi), 0, 0)
+                  C.ptr(vi), 0, EVENT_LAYOUTS[fwd.get("event_layout", "ray")])
     vgrad = None
     if scene.num_views > 1:
         vgrad = torch.empty((scene.num_views, n, 4), dtype=torch.float32, device=dev)

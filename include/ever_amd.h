@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 4
+#define EVER_ABI_VERSION 5
 
 /* status codes */
 #define EVER_OK 0
@@ -90,14 +90,21 @@ typedef struct ever_rays {
    * t > 0 = one block per wave slot of the GPU, each fetching t consecutive 64-ray tiles at a time from a counter
    * until the batch is done. Results do not depend on it. */
   int32_t tile;
-  int32_t reserved;
+  /* Event-list layout (not in the reference, which has only the first): EVER_EVENTS_BY_RAY puts event k of ray r
+   * at tri_collection[r + k*R]; EVER_EVENTS_BY_SLOT puts it at [s + k*R], s = the ray's position in the forward's
+   * ray_order, so that the 64 rays a wave traces together write (and the backward reads) one contiguous row per
+   * event. ever_trace_backward must be given the value the forward used. */
+  int32_t event_layout;
 } ever_rays;
+
+#define EVER_EVENTS_BY_RAY 0
+#define EVER_EVENTS_BY_SLOT 1
 
 typedef struct ever_forward_out {
   float* color;            /* (R,5): R,G,B, opacity, distortion */
   float* states;           /* (R,16) final SplineState (structs.h:41-54) */
   int32_t* iters;          /* (R) events processed */
-  int32_t* tri_collection; /* (R*max_iters), event id 2*prim+entry at [ray + k*R]; nullable */
+  int32_t* tri_collection; /* (R*max_iters), event id 2*prim+entry (layout: ever_rays.event_layout); nullable */
   int32_t* touch_count;    /* (N) events per primitive (exact); nullable; zeroed by the call */
   float* initial_drgb;     /* (R,4) density/colour already open at the ray start */
   double* ray_state64;     /* (R,6) FP64 final accumulators, read by ever_trace_backward */

@@ -28,7 +28,7 @@ def test_every_header_function_is_exported_and_bound():
 
 def test_abi_version_and_struct_sizes():
     L = C.lib()
-    assert L.ever_abi_version() == 4
+    assert L.ever_abi_version() == 5
     sizes = (ctypes.c_int64 * 4)()
     assert L.ever_struct_sizes(sizes) == 0
     assert list(sizes) == [ctypes.sizeof(C.Prims), ctypes.sizeof(C.Rays), ctypes.sizeof(C.ForwardOut),
@@ -69,6 +69,11 @@ def test_invalid_arguments_return_status_codes():
     assert L.ever_trace_forward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r), None,
                                 None) == 1
     assert b"traversal" in L.ever_last_error()
+    r.traversal = 0
+    r.event_layout = 2
+    assert L.ever_trace_forward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r), None,
+                                None) == 1
+    assert b"event_layout" in L.ever_last_error()
     assert L.ever_eval_sh(None, None, None, 5, 1, 4, None, None) == 1
 
 

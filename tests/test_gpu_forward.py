@@ -138,3 +138,34 @@ def test_clustered_needles_subset_matches_oracle():
     ref = oracle.forward(*args, o.double().numpy()[idx], d.double().numpy()[idx], 0.0, 100.0, use_bvh=True)
     np.testing.assert_allclose(_np(out["color"])[idx, :4], ref["color"][:, :4], atol=1e-4, rtol=1e-4)
     assert (_np(out["iters"])[idx] == ref["iters"]).mean() > 0.999
+
+
+@pytest.mark.parametrize("traversal", ["ray", "wave"])
+def test_events_by_slot_are_the_by_ray_events_permuted(traversal):
+    # a batch large enough to be sorted (processing order != ray order); by-slot lists are the by-ray lists with
+    # columns permuted by ray_order, every other output is unchanged and so is the adjoint that reads them
+    print(f"case: events by slot vs by ray, {traversal}", flush=True)
+    sc = scenes.clustered_scene(20_000, 3, seed=21)
+    o, d = scenes.pinhole_rays(320, 240, (0.0, 0.0, -4.0))
+    D = torch.device("cuda:0")
+    t = [sc[k].to(D).contiguous() for k in KEYS]
+    scene = T.Scene(*t, color_mode=1, sh_origin=(0.0, 0.0, -4.0), sh_degree=3, activation=1).build()
+    og, dg = o.to(D).contiguous(), d.to(D).contiguous()
+    a = T.trace_forward(scene, og, dg, 0.0, 100.0, 500, traversal=traversal)
+    b = T.trace_forward(scene, og, dg, 0.0, 100.0, 500, traversal=traversal, event_layout="slot",
+                        count_touches=False)
+    nr = o.shape[0]
+    order = a["ray_order"].long()
+    assert torch.equal(order, b["ray_order"].long())
+    assert not torch.equal(order, torch.arange(nr, device=D))
+    for k in ("color", "iters", "states", "ray_state64", "initial_drgb"):
+        assert torch.equal(a[k], b[k]), k
+    assert b["touch_count"] is None
+    ta, tb = a["tri_collection"].view(500, nr), b["tri_collection"].view(500, nr)
+    mask = torch.arange(500, device=D)[:, None] < a["iters"][order][None, :]
+    assert torch.equal(ta[:, order][mask], tb[mask])
+    g = scenes.upstream_grad(nr, seed=5).to(D)
+    ga = T.trace_backward(scene, og, dg, 0.0, 100.0, 500, a, g)
+    gb = T.trace_backward(scene, og, dg, 0.0, 100.0, 500, b, g)
+    for k in ga:  # per-primitive sums are float atomics: equal up to summation order
+        assert float((ga[k] - gb[k]).norm()) <= 1e-6 * float(ga[k].norm()) + 1e-12, k
