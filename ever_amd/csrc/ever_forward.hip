@@ -337,12 +337,16 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
                      fmaf(RC.z, d.z, fmaf(RC.y, d.y, RC.x * d.x)));
   const float aa = dot3_fma(rdn, rdn);
   const float bp = -dot3_fma(ocn, rdn);
-  const float ba = bp / aa;
+  // hardware reciprocal and square root (each within 1 ulp) instead of the IEEE sequences: these float roots
+  // only order the events and place the cursor (the integrator recomputes every depth in FP64), and both tree
+  // walks share this function, so their event lists stay identical
+  const float ia = __builtin_amdgcn_rcpf(aa);
+  const float ba = bp * ia;
   const f3 l = mk3(fmaf(ba, rdn.x, ocn.x), fmaf(ba, rdn.y, ocn.y), fmaf(ba, rdn.z, ocn.z));
   const float h = aa * (1.f - dot3_fma(l, l));
   const float cc = dot3_fma(ocn, ocn) - 1.f;
-  const float qq = fmaf(sgnf(bp), sqrtf(fmaxf(h, 0.f)), bp);
-  const float r0 = cc / qq, r1 = qq / aa;
+  const float qq = fmaf(sgnf(bp), __builtin_amdgcn_sqrtf(fmaxf(h, 0.f)), bp);
+  const float r0 = cc * __builtin_amdgcn_rcpf(qq), r1 = qq * ia;
   const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
   const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
   const bool hit = h >= 0.f && !(t1 < S.cur_t);
@@ -606,7 +610,9 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
 #pragma unroll
     for (int k = 0; k < K; ++k) ev[k][lane] = kNoEvent;
     uint64_t kmax = kNoEvent;
-    const float t_lo = alive ? S.cur_t : INFINITY;  // a finished lane never enters a box
+    float t_lo = alive ? S.cur_t : INFINITY;  // a finished lane never enters a box
+    // kept opaque: otherwise the select is folded into every box test below (two extra selects per child)
+    asm volatile("" : "+v"(t_lo));
     int32_t stk0 = 0, stk1 = 0;                     // wave stack: entry i in lane i % 64 of stk(i / 64)
     int sp = 0;
     int32_t node = root;
@@ -621,9 +627,10 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4 lx = ld_uniform4(nd, ex), hx = ld_uniform4(nd, 1 - ex), ly = ld_uniform4(nd, 2 + ey),
                      hy = ld_uniform4(nd, 3 - ey), lz = ld_uniform4(nd, 4 + ez), hz = ld_uniform4(nd, 5 - ez),
                      cf = ld_uniform4(nd, 6);
-        const float t_hi =
+        float t_hi =
             alive ? fminf(kBounded ? fminf(A.tmax, S.bound) : A.tmax, __uint_as_float((uint32_t)(kmax >> 32)))
                   : -INFINITY;
+        asm volatile("" : "+v"(t_hi));
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
         const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
@@ -649,10 +656,12 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
             tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
             tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
           }
-          const bool h = tn <= tf;
-          const bool any = used[k] && __builtin_amdgcn_ballot_w64(h) != 0;
-          const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)(h ? __float_as_uint(tn) : 0x7f800000u),
-                                                                  leader);
+          // the lanes entering the box as one mask (ordered compare: a NaN bound enters nothing); the leader's
+          // entry distance for the order of the children, +inf where the leader does not enter
+          const uint64_t hm = __builtin_amdgcn_fcmpf(tn, tf, 5 /* ordered <= */);
+          const bool any = used[k] && hm != 0;
+          const uint32_t tl = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tn), leader);
+          const uint32_t lk = ((hm >> leader) & 1u) ? tl : 0x7f800000u;
           key[k] = any ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
           nhit += any ? 1 : 0;
         }
