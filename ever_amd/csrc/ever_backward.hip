@@ -120,13 +120,24 @@ __global__ void k_invert4(const float* __restrict__ w, int64_t nw, float* __rest
     for (int j = 0; j < 4; ++j) out[16 * k + 4 * i + j] = (float)a[i][j + 4];
 }
 
-// safe root of one event (FP64), value only
-__device__ __forceinline__ double event_t(const BwdArgs& A, d3 org, f3 dir, int32_t id) {
-  const int64_t p = id >> 1;
+// One event's primitive and safe roots (FP64 depth, FP32 intermediates for the adjoint). The loop below needs
+// them twice, as the depth of the previous event and then as the event whose adjoint is taken; they are computed
+// once and carried over.
+struct EventGeo {
   SafeRoots S;
-  float4 q = reinterpret_cast<const float4*>(A.quats)[p];
-  safe_roots(org - tod(ld3(A.means + 3 * p)), dir, ld3(A.scales + 3 * p), q, S);
-  return safe_root_select(S, id & 1);
+  float4 q;
+  f3 mean, scl, ro;  // ro = (float)(origin - mean)
+  bool ok;           // the ray meets the ellipsoid
+};
+__device__ __forceinline__ double event_geo(const BwdArgs& A, d3 org, f3 dir, int32_t id, EventGeo& G) {
+  const int64_t p = id >> 1;
+  G.q = reinterpret_cast<const float4*>(A.quats)[p];
+  G.mean = ld3(A.means + 3 * p);
+  G.scl = ld3(A.scales + 3 * p);
+  const d3 rod = org - tod(G.mean);
+  G.ro = tof(rod);
+  G.ok = safe_roots(rod, dir, G.scl, G.q, G.S);
+  return safe_root_select(G.S, id & 1);
 }
 
 __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_t col, uint32_t* hkeys,
@@ -169,13 +180,15 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
 
   const int32_t view = A.vcol ? A.view_index[r] : 0;
   int32_t id = A.tri[col + (int64_t)(E - 1) * A.nr];
-  double tk = event_t(A, org, dir, id);
+  EventGeo G;
+  double tk = event_geo(A, org, dir, id, G);
   for (int k = E - 1; k >= 0; --k) {
     int32_t id_prev = 0;
     double tprev = 0.0;
+    EventGeo Gp;
     if (k > 0) {
       id_prev = A.tri[col + (int64_t)(k - 1) * A.nr];
-      tprev = event_t(A, org, dir, id_prev);
+      tprev = event_geo(A, org, dir, id_prev, Gp);
     }
     const int64_t p = id >> 1;
     const int face = id & 1;
@@ -198,6 +211,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     const float m = ladder(mid);
     const float csxbf = csxa - w, csybf = csya - w * m;
     const float R1 = (float)Dbf1, R2 = (float)Dbf2, R3 = (float)Dbf3;
+    // IEEE divisions here: gradients at high optical depth are sensitive to this step's rounding
     const float rn0 = safe_div(R1, Df), rn1 = safe_div(R2, Df), rn2 = safe_div(R3, Df);
     // adjoint of update (spline-machine.slang:103-138)
     const float wb = gC0 * rn0 + gC1 * rn1 + gC2 * rn2 + dpxb * 2.f * m * csxbf + dpyb * 2.f * csybf + csxb + csyb * m;
@@ -238,15 +252,10 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     const float sg = face ? 1.f : -1.f;
     const float densb = sg * (dirac0 + dirac1 * cr.y + dirac2 * cr.z + dirac3 * cr.w);
     const float colb0 = sg * cr.x * dirac1, colb1 = sg * cr.x * dirac2, colb2 = sg * cr.x * dirac3;
-    const float4 q = reinterpret_cast<const float4*>(A.quats)[p];
-    const f3 mean = ld3(A.means + 3 * p);
-    const f3 scl = ld3(A.scales + 3 * p);
-    const d3 rod = org - tod(mean);
-    const f3 ro = tof(rod);
-    SafeRoots S;
+    const f3 mean = G.mean;
     f3 rob = mk3(0.f, 0.f, 0.f), rdb = mk3(0.f, 0.f, 0.f), sbar = mk3(0.f, 0.f, 0.f);
     float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (safe_roots(rod, dir, scl, q, S)) safe_roots_bwd(S, ro, dir, scl, q, face, tb, rob, rdb, sbar, qb);
+    if (G.ok) safe_roots_bwd(G.S, G.ro, dir, G.scl, G.q, face, tb, rob, rdb, sbar, qb);
     orgb = orgb + rob;
     dirb = dirb + rdb;
     {
@@ -273,6 +282,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     tcarry = tprevb;
     id = id_prev;
     tk = tprev;
+    G = Gp;
   }
   *dinit = make_float4(Db0, Db1, Db2, Db3);
   float* o = A.drayo + 3 * r;

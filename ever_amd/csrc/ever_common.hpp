@@ -48,6 +48,23 @@ __device__ __forceinline__ void bw_safe_div(float a, float b, float g, float& da
 }
 __device__ __forceinline__ float safe_sqrt(float a) { return a < kTiny ? 0.f : sqrtf(a); }
 
+// One safe_div denominator used several times in an adjoint: the reciprocals of b' (b, or kTiny where |b| <
+// kTiny) and of b2' (b*b, or kTiny below it), from the hardware reciprocal (within 1 ulp). With them
+// safe_div(a, b) ~ clip(a * r1) and bw_safe_div(a, b, g) ~ (clip(g * r1), clip(-a * r2 * g)), a few ulp from
+// the divisions; used in gradients only, never in the forward's colours.
+struct SafeRcp {
+  float r1, r2;
+};
+__device__ __forceinline__ SafeRcp safe_rcp(float b) {
+  const float b2 = b * b;
+  return {__builtin_amdgcn_rcpf(fabsf(b) < kTiny ? kTiny : b), __builtin_amdgcn_rcpf(b2 < kTiny ? kTiny : b2)};
+}
+__device__ __forceinline__ float safe_div_r(float a, const SafeRcp& r) { return clipf(a * r.r1, kMinVal, kMaxVal); }
+__device__ __forceinline__ void bw_safe_div_r(float a, const SafeRcp& r, float g, float& da, float& db) {
+  da = clipf(g * r.r1, kMinVal, kMaxVal);
+  db = clipf(-a * r.r2 * g, kMinVal, kMaxVal);
+}
+
 // rotation by the conjugate quaternion of q = (w, x, y, z)  (safe-math.slang:237-244)
 __device__ __forceinline__ f3 qrot(f3 v, float4 q) {
   f3 u = mk3(-q.y, -q.z, -q.w);
@@ -75,12 +92,13 @@ __device__ __forceinline__ f3 l2n(f3 x) {
 __device__ __forceinline__ f3 l2n_bwd(f3 x, f3 yb) {
   float m = dot3(x, x);
   float n = sqrtf(fmaxf(m, kTiny));
+  const SafeRcp rn = safe_rcp(n);
   float ax, ay, az, b0, b1, b2;
-  bw_safe_div(x.x, n, yb.x, ax, b0);
-  bw_safe_div(x.y, n, yb.y, ay, b1);
-  bw_safe_div(x.z, n, yb.z, az, b2);
+  bw_safe_div_r(x.x, rn, yb.x, ax, b0);
+  bw_safe_div_r(x.y, rn, yb.y, ay, b1);
+  bw_safe_div_r(x.z, rn, yb.z, az, b2);
   f3 xb = mk3(ax, ay, az);
-  if (m > kTiny) xb = xb + ((b0 + b1 + b2) / n) * x;
+  if (m > kTiny) xb = xb + ((b0 + b1 + b2) * __builtin_amdgcn_rcpf(n)) * x;
   return xb;
 }
 
@@ -158,6 +176,12 @@ __device__ __forceinline__ double safe_div_d(double a, double b) {
   const double r = a / (fabs(b) < (double)kTiny ? (double)kTiny : b);
   return fmax(fmin(r, (double)kMaxVal), (double)kMinVal);
 }
+// safe_div_d by way of the (correctly rounded) reciprocal of the guarded denominator, for a denominator shared by
+// several quotients: within 1 ulp (FP64) of safe_div_d
+__device__ __forceinline__ double safe_rcp_d(double b) { return 1.0 / (fabs(b) < (double)kTiny ? (double)kTiny : b); }
+__device__ __forceinline__ double safe_mul_d(double a, double rb) {
+  return fmax(fmin(a * rb, (double)kMaxVal), (double)kMinVal);
+}
 __device__ __forceinline__ d3 qrot_d(d3 v, float4 q) {
   d3 u = mkd(-q.y, -q.z, -q.w);
   d3 t = 2.0 * crossd(u, v);
@@ -176,13 +200,16 @@ struct SafeRoots {
 __device__ __forceinline__ bool safe_roots(d3 ro, f3 rd, f3 s, float4 q, SafeRoots& S) {
   const d3 rd_rot = qrot_d(tod(rd), q);
   const double nrm = sqrt(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
-  const d3 Trd = mkd(safe_div_d(rd_rot.x, nrm), safe_div_d(rd_rot.y, nrm), safe_div_d(rd_rot.z, nrm));
+  const double rnrm = safe_rcp_d(nrm);
+  const d3 Trd = mkd(safe_mul_d(rd_rot.x, rnrm), safe_mul_d(rd_rot.y, rnrm), safe_mul_d(rd_rot.z, rnrm));
   const d3 Tro = qrot_d(ro, q);
-  const d3 ocn = mkd(safe_div_d(Tro.x, s.x), safe_div_d(Tro.y, s.y), safe_div_d(Tro.z, s.z));
-  const d3 rdn = mkd(safe_div_d(Trd.x, s.x), safe_div_d(Trd.y, s.y), safe_div_d(Trd.z, s.z));
+  const double rsx = safe_rcp_d(s.x), rsy = safe_rcp_d(s.y), rsz = safe_rcp_d(s.z);
+  const d3 ocn = mkd(safe_mul_d(Tro.x, rsx), safe_mul_d(Tro.y, rsy), safe_mul_d(Tro.z, rsz));
+  const d3 rdn = mkd(safe_mul_d(Trd.x, rsx), safe_mul_d(Trd.y, rsy), safe_mul_d(Trd.z, rsz));
   const double a = dotd(rdn, rdn);
   const double bp = -dotd(ocn, rdn);
-  const double ba = safe_div_d(bp, a);
+  const double ra = safe_rcp_d(a);
+  const double ba = safe_mul_d(bp, ra);
   const d3 l = ocn + ba * rdn;
   const double omll = 1.0 - dotd(l, l);
   const double h = a * omll;
@@ -207,7 +234,7 @@ __device__ __forceinline__ bool safe_roots(d3 ro, f3 rd, f3 s, float4 q, SafeRoo
   const double sh = h < (double)kTiny ? 0.0 : sqrt(h);
   const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
   const double qq = bp + sb * sh;
-  const double r0 = safe_div_d(c, qq), r1 = safe_div_d(qq, a);
+  const double r0 = safe_div_d(c, qq), r1 = safe_mul_d(qq, ra);
   S.sh = (float)sh;
   S.sb = (float)sb;
   S.qq = (float)qq;
@@ -232,7 +259,8 @@ __device__ __forceinline__ void safe_roots_bwd(const SafeRoots& S, f3 ro, f3 rd,
   bw_safe_div(S.c, S.qq, r0b, da, db);
   cb = da;
   qqb = db;
-  bw_safe_div(S.qq, S.a, r1b, da, db);
+  const SafeRcp ra = safe_rcp(S.a);
+  bw_safe_div_r(S.qq, ra, r1b, da, db);
   qqb += da;
   ab += db;
   bpb = qqb;
@@ -242,18 +270,19 @@ __device__ __forceinline__ void safe_roots_bwd(const SafeRoots& S, f3 ro, f3 rd,
   f3 ocnb = (2.f * cb) * S.ocn + lb;
   float bab = dot3(lb, S.rdn);
   f3 rdnb = S.ba * lb;
-  bw_safe_div(S.bp, S.a, bab, da, db);
+  bw_safe_div_r(S.bp, ra, bab, da, db);
   bpb += da;
   ab += db;
   ocnb = ocnb + (-bpb) * S.rdn;
   rdnb = rdnb + (-bpb) * S.ocn + (2.f * ab) * S.rdn;
   f3 Trob, Trdb;
-  bw_safe_div(S.Tro.x, s.x, ocnb.x, da, db); Trob.x = da; sbar.x += db;
-  bw_safe_div(S.Tro.y, s.y, ocnb.y, da, db); Trob.y = da; sbar.y += db;
-  bw_safe_div(S.Tro.z, s.z, ocnb.z, da, db); Trob.z = da; sbar.z += db;
-  bw_safe_div(S.Trd.x, s.x, rdnb.x, da, db); Trdb.x = da; sbar.x += db;
-  bw_safe_div(S.Trd.y, s.y, rdnb.y, da, db); Trdb.y = da; sbar.y += db;
-  bw_safe_div(S.Trd.z, s.z, rdnb.z, da, db); Trdb.z = da; sbar.z += db;
+  const SafeRcp rx = safe_rcp(s.x), ry = safe_rcp(s.y), rz = safe_rcp(s.z);
+  bw_safe_div_r(S.Tro.x, rx, ocnb.x, da, db); Trob.x = da; sbar.x += db;
+  bw_safe_div_r(S.Tro.y, ry, ocnb.y, da, db); Trob.y = da; sbar.y += db;
+  bw_safe_div_r(S.Tro.z, rz, ocnb.z, da, db); Trob.z = da; sbar.z += db;
+  bw_safe_div_r(S.Trd.x, rx, rdnb.x, da, db); Trdb.x = da; sbar.x += db;
+  bw_safe_div_r(S.Trd.y, ry, rdnb.y, da, db); Trdb.y = da; sbar.y += db;
+  bw_safe_div_r(S.Trd.z, rz, rdnb.z, da, db); Trdb.z = da; sbar.z += db;
   qrot_bwd(ro, q, Trob, rob, qb);
   f3 rrb = l2n_bwd(S.rd_rot, Trdb);
   qrot_bwd(rd, q, rrb, rdb, qb);

@@ -26,6 +26,7 @@ def assert_grad_close(a, r, name, l2=1e-4, mx=1e-3):
     scale = max(np.abs(r).max(), 1e-12)
     e_l2 = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-12)
     e_mx = np.abs(a - r).max() / scale
+    print(f"{name}: L2 rel {e_l2:.2e}, max rel {e_mx:.2e}", flush=True)
     assert e_l2 <= l2 and e_mx <= mx, f"{name}: L2 rel {e_l2:.2e}, max rel {e_mx:.2e}"
 
 
