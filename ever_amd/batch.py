@@ -118,7 +118,7 @@ def workload_key(num_prims, num_rays, num_views, device):
 def autotune(make_step, candidates=None, reps=3, key=None, use_cache=True):
     """Times `make_step(kbuf, tile).run()` (build + forward + backward) for each candidate (kbuf, tile) and returns
     (best, {candidate: ms}); with a key, results are cached in a JSON file and reused."""
-    candidates = candidates or [(k, t) for k in (8, 16, 32) for t in (0, 1, 2, 4)]
+    candidates = candidates or [(k, t) for k in (8, 16, 19, 32) for t in (0, 1, 2, 4)]
     path = _cache_path()
     cache = {}
     if key and use_cache and os.path.exists(path):

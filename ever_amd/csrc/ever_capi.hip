@@ -53,7 +53,8 @@ int check_rays(const ever_rays* R) {
   if (R->num_rays < 0) return fail(EVER_ERR_INVALID, "num_rays must be >= 0");
   if (R->num_rays > 0 && (!R->origins || !R->directions)) return fail(EVER_ERR_INVALID, "ray arrays are null");
   if (R->max_iters < 0) return fail(EVER_ERR_INVALID, "max_iters must be >= 0");
-  if (R->kbuf != 8 && R->kbuf != 16 && R->kbuf != 32) return fail(EVER_ERR_INVALID, "kbuf must be 8, 16 or 32");
+  if (R->kbuf != 8 && R->kbuf != 16 && R->kbuf != 19 && R->kbuf != 32)
+    return fail(EVER_ERR_INVALID, "kbuf must be 8, 16, 19 or 32");
   if (R->traversal != EVER_TRAVERSAL_AUTO && R->traversal != EVER_TRAVERSAL_RAY && R->traversal != EVER_TRAVERSAL_WAVE)
     return fail(EVER_ERR_INVALID, "traversal must be EVER_TRAVERSAL_AUTO, _RAY or _WAVE");
   if (R->tile < 0 || R->tile > 64) return fail(EVER_ERR_INVALID, "tile must be in [0, 64]");

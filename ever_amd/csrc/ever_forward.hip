@@ -579,14 +579,20 @@ __device__ __forceinline__ float4 ld_uniform4(const float4* p, int64_t i) {
 // The k-buffer of a wave lives in LDS (entry k of lane l at ev[k][l]); only its last entry, the culling bound,
 // is kept in a register. An insertion reads the lane's K entries, bubbles the event in and writes them back; it
 // runs only when some lane of the wave has an event below its bound.
+constexpr int pow2_floor(int n) { return n < 2 ? 1 : 2 * pow2_floor(n / 2); }
+
 template <int K>
 __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
   if (!(v < kmax)) return;
   // position = number of entries below v (binary search over the sorted entries; < K since v < the last)
+  // (power-of-two steps over the first K - 1 entries; any K)
+  constexpr int P = pow2_floor(K - 1);
   int lo = 0;
 #pragma unroll
-  for (int step = K / 2; step >= 1; step >>= 1)
-    if (ev[lo + step - 1][lane] < v) lo += step;
+  for (int step = P; step >= 1; step >>= 1) {
+    const int i = lo + step - 1;
+    if (i < K - 1 && ev[i][lane] < v) lo += step;
+  }
   if (ev[lo][lane] == v) return;  // the fragments of one primitive report identical events: keep one
   // shift the entries from the position up by one (events mostly arrive near-first, so few move)
   for (int j = K - 1; j > lo; --j) ev[j][lane] = ev[j - 1][lane];
@@ -722,7 +728,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
 }
 
 template <int K, int kFlags>  // kFlags: 1 proposal-bounded, 2 several views, 4 persistent
-__global__ __launch_bounds__(kWaveBlock) void k_trace_forward_wave(FwdArgs A) {
+__global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : 4) void k_trace_forward_wave(FwdArgs A) {
   constexpr bool kBounded = (kFlags & 1) != 0, kViews = (kFlags & 2) != 0, kPersistent = (kFlags & 4) != 0;
   const int lane = threadIdx.x;
   __shared__ uint64_t ev[K][kWaveBlock];
@@ -779,8 +785,10 @@ static void (*pick_flags(int flags))(FwdArgs) {
 
 template <bool kWave>
 static void (*pick_kernel(int kbuf, int flags))(FwdArgs) {
-  return kbuf == 8 ? pick_flags<8, kWave>(flags) : kbuf == 32 ? pick_flags<32, kWave>(flags)
-                                                               : pick_flags<16, kWave>(flags);
+  return kbuf == 8    ? pick_flags<8, kWave>(flags)
+         : kbuf == 19 ? pick_flags<19, kWave>(flags)
+         : kbuf == 32 ? pick_flags<32, kWave>(flags)
+                      : pick_flags<16, kWave>(flags);
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,

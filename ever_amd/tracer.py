@@ -13,6 +13,9 @@ import torch
 from . import _capi as C
 
 BUFFER_SIZE = 16  # events gathered per traversal (shaders.slang:18)
+# k-buffer sizes with compiled kernels; results do not depend on the size (19 is the largest whose per-wave LDS
+# buffer, 19 x 64 x 8 B plus a 256 B prefetch area, still lets 16 waves share a CU's 160 KB)
+KBUF_SIZES = (8, 16, 19, 32)
 _KBUF = int(os.environ.get("EVER_KBUF", BUFFER_SIZE))
 
 
@@ -58,10 +61,10 @@ def _t1(tok):
 
 
 def set_kbuf(k):
-    """Select the k-buffer size (8, 16 or 32) used by subsequent traces."""
+    """Select the k-buffer size (8, 16, 19 or 32) used by subsequent traces."""
     global _KBUF
-    if k not in (8, 16, 32):
-        raise ValueError("kbuf must be 8, 16 or 32")
+    if k not in KBUF_SIZES:
+        raise ValueError("kbuf must be 8, 16, 19 or 32")
     _KBUF = k
 
 

@@ -77,7 +77,7 @@ typedef struct ever_rays {
   float tmin;
   float tmax;
   int32_t max_iters;
-  int32_t kbuf; /* events gathered per traversal: 8, 16 (reference) or 32 */
+  int32_t kbuf; /* events gathered per traversal: 8, 16 (reference), 19 or 32; results do not depend on it */
   int32_t traversal; /* EVER_TRAVERSAL_*: who walks the tree, one lane per ray or the 64 rays of a wave */
   /* (R) proposed termination depth per ray, nullable. Depths are measured along the unit direction from
    * origin + tmin * direction. A ray first gathers only events up to its proposal (nodes beyond it are not

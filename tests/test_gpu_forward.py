@@ -37,7 +37,7 @@ def test_config1_matches_golden_oracle(traversal):
     np.testing.assert_array_equal(_np(out["touch_count"]), z["touch_count"])
 
 
-@pytest.mark.parametrize("kbuf", [8, 16, 32])
+@pytest.mark.parametrize("kbuf", [8, 16, 19, 32])
 def test_kbuffer_sizes_agree(kbuf):
     print(f"case: kbuf {kbuf}", flush=True)
     sc, o, d = scenes.config1()
