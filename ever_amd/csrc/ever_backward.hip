@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
 struct EpiArgs {
   const float* acc;
   const float* means;
-  const float* feat;
+  const float4* colr;  // (density, colour) of each primitive as the scene build computed it (view 0)
   int64_t n;
   int d;
   int color_mode, sh_degree, activation;
@@ -381,6 +381,7 @@ struct EpiArgs {
   // several views: view origins (V,3) and the per-view colour gradients (V,N,4), turned in place into the
   // colour-space gradients before the SH basis
   const float* vorig;
+  const float4* vcol;  // (V,N) per-view (density, colour) of the build
   int nv;
   float* vgrad;
 };
@@ -471,20 +472,14 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
               g2 = clipf(kShC0 * a3.y, -1e3f, 1e3f);
   float* df = E.dfeat + (int64_t)3 * E.d * p;
   if (E.color_mode == EVER_COLOR_SH) {
-    // chain through RGB2SH (1/SH_C0) and the optional softplus; the SH basis is applied by k_sh_feature_grads
-    const f3 mean = ld3(E.means + 3 * p);
+    // chain through RGB2SH (1/SH_C0) and the optional softplus (its derivative from the colour the build
+    // stored, so the SH features are not read again); the SH basis is applied by k_sh_feature_grads
     float gr[3] = {g0 / kShC0, g1 / kShC0, g2 / kShC0};
     if (E.activation == 1) {
-      const f3 dir = sh_dir(mean, mk3(E.ox, E.oy, E.oz));
-      float bas[16];
-      sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
-      const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
-      const float* f = E.feat + (int64_t)3 * E.d * p;
-      for (int ch = 0; ch < 3; ++ch) {
-        float x = 0.5f;
-        for (int k = 0; k < nc; ++k) x += bas[k] * f[3 * k + ch];
-        gr[ch] *= softplus10_grad(x);
-      }
+      const float4 c = E.colr[p];
+      gr[0] *= softplus10_grad_from_value(c.y);
+      gr[1] *= softplus10_grad_from_value(c.z);
+      gr[2] *= softplus10_grad_from_value(c.w);
     }
     // bw_sh_kernel skips primitives whose colour gradient is below 1e-8 in every channel (sh_kernel.slang:167-170)
     const bool skip = fabsf(gr[0]) < 1e-8f && fabsf(gr[1]) < 1e-8f && fabsf(gr[2]) < 1e-8f;
@@ -496,16 +491,10 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
       float g[3] = {clipf(kShC0 * cg.x, -1e3f, 1e3f) / kShC0, clipf(kShC0 * cg.y, -1e3f, 1e3f) / kShC0,
                     clipf(kShC0 * cg.z, -1e3f, 1e3f) / kShC0};
       if (E.activation == 1) {
-        const f3 dir = sh_dir(mean, ld3(E.vorig + 3 * v));
-        float bas[16];
-        sh_basis(E.sh_degree, dir.x, dir.y, dir.z, bas);
-        const int nc = (E.sh_degree + 1) * (E.sh_degree + 1);
-        const float* f = E.feat + (int64_t)3 * E.d * p;
-        for (int ch = 0; ch < 3; ++ch) {
-          float x = 0.5f;
-          for (int k = 0; k < nc; ++k) x += bas[k] * f[3 * k + ch];
-          g[ch] *= softplus10_grad(x);
-        }
+        const float4 c = E.vcol[(int64_t)v * E.n + p];
+        g[0] *= softplus10_grad_from_value(c.y);
+        g[1] *= softplus10_grad_from_value(c.z);
+        g[2] *= softplus10_grad_from_value(c.w);
       }
       const bool sk = fabsf(g[0]) < 1e-8f && fabsf(g[1]) < 1e-8f && fabsf(g[2]) < 1e-8f;
       *gv4 = sk ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(g[0], g[1], g[2], 0.f);
@@ -591,7 +580,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     EpiArgs E;
     E.acc = acc;
     E.means = P.means;
-    E.feat = P.features;
+    E.colr = at<float4>(ws, L.colr);
     E.n = n;
     E.d = P.feature_size;
     E.color_mode = P.color_mode;
@@ -611,6 +600,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     E.dmeans2D = nw > 0 ? B.dmeans2D : nullptr;
     E.grgb = at<float>(B.workspace, BL.grgb);
     E.vorig = views ? P.view_origins : nullptr;
+    E.vcol = views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
     E.nv = views ? P.num_views : 0;
     E.vgrad = views ? B.view_color_grads : nullptr;
     hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);

@@ -449,19 +449,29 @@ __device__ __forceinline__ Box range_union(int32_t lo, int32_t hi, const float* 
 //   f4[0] = lo.x of children 0..3, f4[1] = hi.x, f4[2] = lo.y, f4[3] = hi.y, f4[4] = lo.z, f4[5] = hi.z,
 //   f4[6] = child refs (internal: binary node index of the child's record; leaf: ~primitive; empty: kEmptyChild)
 // Only records reachable from the root (binary node 0) are visited.
-__device__ __forceinline__ Box child_box(int32_t c, const int2* range, const float* sbox, const float* pre,
-                                         const float* suf, const float* tab, int64_t nb) {
-  if (c < 0) return box_load(sbox + 6 * (int64_t)(~c));
-  const int2 r = range[c];
-  return range_union(r.x, r.y, sbox, pre, suf, tab, nb);
+// Box of every binary node (the union over its leaf range), once per node, into ibox: the records below gather
+// their four child boxes from it instead of evaluating four range unions each.
+__global__ __launch_bounds__(256) void k_binbox(const int2* __restrict__ range, const float* __restrict__ sbox,
+                                                const float* __restrict__ pre, const float* __restrict__ suf,
+                                                const float* __restrict__ tab, const uint32_t* __restrict__ header,
+                                                float* __restrict__ ibox, int64_t nb) {
+  const int64_t n = header[9];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int2 r = range[i];
+  const Box b = range_union(r.x, r.y, sbox, pre, suf, tab, nb);
+  float* o = ibox + 6 * i;
+  o[0] = b.lx; o[1] = b.ly; o[2] = b.lz; o[3] = b.hx; o[4] = b.hy; o[5] = b.hz;
+}
+
+__device__ __forceinline__ Box child_box(int32_t c, const float* sbox, const float* ibox) {
+  return c < 0 ? box_load(sbox + 6 * (int64_t)(~c)) : box_load(ibox + 6 * (int64_t)c);
 }
 
 __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2, float4* __restrict__ nodes4,
-                                               const int2* __restrict__ range, const float* __restrict__ sbox,
-                                               const float* __restrict__ pre, const float* __restrict__ suf,
-                                               const float* __restrict__ tab, const uint32_t* __restrict__ header,
-                                               const uint32_t* __restrict__ order, const uint32_t* __restrict__ fprim,
-                                               int64_t nb) {
+                                               const float* __restrict__ sbox, const float* __restrict__ ibox,
+                                               const uint32_t* __restrict__ header,
+                                               const uint32_t* __restrict__ order, const uint32_t* __restrict__ fprim) {
   const int64_t n = header[9];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
@@ -477,9 +487,9 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
   } else {
     const float4 ge = nodes2[c0];
     const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
-    ba0 = child_box(g0, range, sbox, pre, suf, tab, nb);
+    ba0 = child_box(g0, sbox, ibox);
     ra0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
-    ba1 = child_box(g1, range, sbox, pre, suf, tab, nb);
+    ba1 = child_box(g1, sbox, ibox);
     ra1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
   }
   if (c1 < 0) {
@@ -488,9 +498,9 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
   } else {
     const float4 ge = nodes2[c1];
     const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
-    bb0 = child_box(g0, range, sbox, pre, suf, tab, nb);
+    bb0 = child_box(g0, sbox, ibox);
     rb0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
-    bb1 = child_box(g1, range, sbox, pre, suf, tab, nb);
+    bb1 = child_box(g1, sbox, ibox);
     rb1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
   }
   const bool two = c0 >= 0;  // child 0 took two slots
@@ -576,8 +586,10 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   for (int k = 1; k < L.tab_levels; ++k)
     hipLaunchKernelGGL(k_table_level, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab + (int64_t)(k - 1) * 6 * nb,
                        tab + (int64_t)k * 6 * nb, nb, (int64_t)1 << (k - 1));
-  hipLaunchKernelGGL(k_node4, dim3(grid_for(S, 256)), dim3(256), 0, st, nodes, nodes4, range, sbox, pre, suf, tab,
-                     header, v0, fprim, nb);
+  // the fragment boxes (aabb) are dead once the leaf blocks are built: their region holds the binary-node boxes
+  float* ibox = aabb;
+  hipLaunchKernelGGL(k_binbox, dim3(grid_for(S, 256)), dim3(256), 0, st, range, sbox, pre, suf, tab, header, ibox, nb);
+  hipLaunchKernelGGL(k_node4, dim3(grid_for(S, 256)), dim3(256), 0, st, nodes, nodes4, sbox, ibox, header, v0, fprim);
   return hipGetLastError();
 }
 

@@ -157,6 +157,9 @@ __device__ __forceinline__ float softplus10_grad(float x) {
   float bx = 10.f * x;
   return bx > 20.f ? 1.f : 1.f / (1.f + expf(-bx));
 }
+// The same derivative from the activated value y = softplus10(x): sigmoid(10 x) = 1 - exp(-10 y), evaluated as
+// -expm1(-10 y) (a few ulp; above the threshold, y = x > 2 and it rounds to 1 as the derivative is there)
+__device__ __forceinline__ float softplus10_grad_from_value(float y) { return -expm1f(-10.f * y); }
 
 // FP64 vector helpers for the root evaluation of the backward.
 struct d3 {

@@ -25,7 +25,7 @@ constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 
 // Leaves are fragments: each primitive owns kMaxFragments slots (S = kMaxFragments * N); unused slots carry the
 // sort key 0xffffffff and end behind the L used leaves (L is counted on the device, header[9]).
 //   header   : uint32[16]  (0..5 centroid bounds as ordered ints, 8 = start-inside candidate count, 9 = L)
-//   aabb     : float[S*6]  fragment boxes (slot order)
+//   aabb     : float[S*6]  fragment boxes (slot order); once the leaf blocks are built, the binary-node boxes
 //   fprim    : uint32[S]   slot -> primitive
 //   keys0/1, vals0/1 : uint32[S] radix-sort ping-pong (vals0 ends as sorted leaf -> slot)
 //   hist     : uint32[256 * tiles + 256] (digit-major tile counts, then digit totals)
