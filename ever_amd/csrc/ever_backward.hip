@@ -120,20 +120,34 @@ __global__ void k_invert4(const float* __restrict__ w, int64_t nw, float* __rest
     for (int j = 0; j < 4; ++j) out[16 * k + 4 * i + j] = (float)a[i][j + 4];
 }
 
+// The primitive data one event needs, loaded an event ahead of its use (the loop below walks each ray's events
+// back to front, and a load issued one iteration early has that iteration's arithmetic to arrive).
+struct EventRaw {
+  float4 q, cr;  // quaternion; (density, colour) of the event's view
+  f3 mean, scl;
+};
+__device__ __forceinline__ void event_load(const BwdArgs& A, int32_t id, int32_t view, EventRaw& R) {
+  const int64_t p = id >> 1;
+  R.q = reinterpret_cast<const float4*>(A.quats)[p];
+  R.cr = A.vcol ? A.vcol[(int64_t)view * A.n + p] : A.colr[p];
+  R.mean = ld3(A.means + 3 * p);
+  R.scl = ld3(A.scales + 3 * p);
+}
+
 // One event's primitive and safe roots (FP64 depth, FP32 intermediates for the adjoint). The loop below needs
 // them twice, as the depth of the previous event and then as the event whose adjoint is taken; they are computed
 // once and carried over.
 struct EventGeo {
   SafeRoots S;
-  float4 q;
+  float4 q, cr;
   f3 mean, scl, ro;  // ro = (float)(origin - mean)
   bool ok;           // the ray meets the ellipsoid
 };
-__device__ __forceinline__ double event_geo(const BwdArgs& A, d3 org, f3 dir, int32_t id, EventGeo& G) {
-  const int64_t p = id >> 1;
-  G.q = reinterpret_cast<const float4*>(A.quats)[p];
-  G.mean = ld3(A.means + 3 * p);
-  G.scl = ld3(A.scales + 3 * p);
+__device__ __forceinline__ double event_geo(d3 org, f3 dir, int32_t id, const EventRaw& R, EventGeo& G) {
+  G.q = R.q;
+  G.cr = R.cr;
+  G.mean = R.mean;
+  G.scl = R.scl;
   const d3 rod = org - tod(G.mean);
   G.ro = tof(rod);
   G.ok = safe_roots(rod, dir, G.scl, G.q, G.S);
@@ -179,20 +193,36 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   }
 
   const int32_t view = A.vcol ? A.view_index[r] : 0;
-  int32_t id = A.tri[col + (int64_t)(E - 1) * A.nr];
+  // software pipeline over the events k = E-1 .. 0: event ids are read two events ahead, primitive data one
+  // event ahead, so the loads of iteration k are issued during iteration k + 1
+  const int32_t* tcol = A.tri + col;
+  int32_t id = tcol[(int64_t)(E - 1) * A.nr];
+  int32_t id_next = E >= 2 ? tcol[(int64_t)(E - 2) * A.nr] : 0;   // event k - 1
+  int32_t id_next2 = E >= 3 ? tcol[(int64_t)(E - 3) * A.nr] : 0;  // event k - 2
   EventGeo G;
-  double tk = event_geo(A, org, dir, id, G);
+  double tk;
+  {
+    EventRaw R0;
+    event_load(A, id, view, R0);
+    tk = event_geo(org, dir, id, R0, G);
+  }
+  EventRaw Rn;  // raw data of event k - 1
+  if (E >= 2) event_load(A, id_next, view, Rn);
   for (int k = E - 1; k >= 0; --k) {
+    EventRaw Rn2;  // raw data of event k - 2 and the id of event k - 3, for the next iterations
+    int32_t id_next3 = 0;
+    if (k >= 2) event_load(A, id_next2, view, Rn2);
+    if (k >= 3) id_next3 = tcol[(int64_t)(k - 3) * A.nr];
     int32_t id_prev = 0;
     double tprev = 0.0;
     EventGeo Gp;
     if (k > 0) {
-      id_prev = A.tri[col + (int64_t)(k - 1) * A.nr];
-      tprev = event_geo(A, org, dir, id_prev, Gp);
+      id_prev = id_next;
+      tprev = event_geo(org, dir, id_prev, Rn, Gp);
     }
     const int64_t p = id >> 1;
     const int face = id & 1;
-    const float4 cr = A.vcol ? A.vcol[(int64_t)view * A.n + p] : A.colr[p];
+    const float4 cr = G.cr;
     const float hgt = face ? cr.x : -cr.x;
     // reconstruct the state before event k (inverse of the forward's FP64 accumulation)
     const double Dbf0 = Da0 - (double)hgt, Dbf1 = Da1 - (double)(hgt * cr.y), Dbf2 = Da2 - (double)(hgt * cr.z),
@@ -283,6 +313,9 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     id = id_prev;
     tk = tprev;
     G = Gp;
+    Rn = Rn2;
+    id_next = id_next2;
+    id_next2 = id_next3;
   }
   *dinit = make_float4(Db0, Db1, Db2, Db3);
   float* o = A.drayo + 3 * r;
