@@ -52,6 +52,8 @@ def algorithmic_bytes(out, nr, n, d, max_iters):
     The SH prepass (part of the scene build) is counted separately."""
     import torch
     iters = out["iters"].long()
+    if out.get("event_layout") == "slot":  # column s of the event list belongs to ray ray_order[s]
+        iters = iters[out["ray_order"].long()]
     tri = out["tri_collection"].view(max_iters, nr)
     E = int(iters.sum())
     ent = torch.zeros(nr, dtype=torch.long, device=iters.device)
@@ -237,9 +239,10 @@ def main():
         C.check(L.ever_profile_read(which, ctypes.byref(tot), ctypes.byref(cnt)), "ever_profile_read")
         kernel_ms[name] = tot.value / max(cnt.value, 1)
 
-    # one untimed forward for the algorithmic byte count of this workload
+    # one untimed forward for the algorithmic byte count of this workload, configured as the step's forward (events
+    # by processing slot, no touch counts); the counts below do not depend on the event-list layout
     scene = T.Scene(*[x.detach() for x in params], color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
-    out = T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters)
+    out = T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters, count_touches=False, event_layout="slot")
     ab = algorithmic_bytes(out, nr, a.prims, 16, a.max_iters)
     fwd_ms, bwd_ms = kernel_ms["forward"], kernel_ms["backward"]
     dom, dom_ms, dom_bytes = ("forward", fwd_ms, ab["fwd"]) if fwd_ms >= bwd_ms else ("backward", bwd_ms, ab["bwd"])
