@@ -226,6 +226,23 @@ struct RayState {
   float bound;  // events beyond this depth are left for a later (plain) traversal; +inf once plain
 };
 
+// A lane of the per-wave walk with no ray: it stays in the wave (the wave's stack keeps entry i in lane i, and a
+// lane that has left the wave would drop the entries written to it) and is never alive (iter = max_iters), so it
+// never enters a box.
+__device__ __forceinline__ void ray_idle(const FwdArgs& A, RayState& S) {
+  S.r = 0;
+  S.e = 0;
+  S.o = S.d = S.oi = mk3(0.f, 0.f, 0.f);
+  S.inv = mk3(1.f, 1.f, 1.f);
+  S.D0 = S.D1 = S.D2 = S.D3 = 0.0;
+  S.t_state = S.logT = 0.0;
+  S.C0 = S.C1 = S.C2 = S.csx = S.csy = S.dpx = S.dpy = 0.f;
+  S.cur_t = 0.f;
+  S.cursor = 0;
+  S.iter = A.max_iters;
+  S.bound = INFINITY;
+}
+
 template <bool kViews>
 __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t slot, RayState& S) {
   S.r = (int32_t)r;
@@ -606,7 +623,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
   const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
-  bool done = false;
+  bool done = false;  // (a lane without a ray is never alive: ray_idle)
 
   while (true) {
     const bool alive = !done && ray_alive(A, S);
@@ -733,18 +750,26 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : 4) void k_trace_forw
   const int lane = threadIdx.x;
   __shared__ uint64_t ev[K][kWaveBlock];
   __shared__ uint32_t pf_scratch[kWaveBlock];
-  // the 64 rays of processing slots [base, base + 64); lanes past the batch take no part
+  // the 64 rays of processing slots [base, base + 64); lanes past the batch stay in the wave without a ray
+  // (ray_idle), so that all 64 lanes hold the wave's stack
   auto trace_tile = [&](int64_t base) {
+    if (base >= A.nr) return;  // the whole wave: uniform
     const int64_t slot = base + lane;
-    if (slot >= A.nr) return;
+    const bool valid = slot < A.nr;
     RayState S;
-    ray_setup<kViews>(A, A.order[slot], slot, S);
-    if (kBounded) ray_bound(A, S);
-    const uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
+    if (valid) {
+      ray_setup<kViews>(A, A.order[slot], slot, S);
+      if (kBounded) ray_bound(A, S);
+    } else {
+      ray_idle(A, S);
+    }
+    // lane 0 always has a ray; rayless lanes take its signs so they do not break the wave's sign uniformity
+    uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
     const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
+    if (!valid) sg = sg0;
     if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
     else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
-    ray_write(A, S);
+    if (base + lane < A.nr) ray_write(A, S);
   };
   if (!kPersistent) {
     trace_tile((int64_t)blockIdx.x * kWaveBlock);

@@ -49,6 +49,16 @@ def test_incoherent_rays_and_partial_wave():
     assert int(a["iters"].max()) > 0
 
 
+def test_lone_ray_in_last_wave():
+    # 64 + 1 rays: the second wave holds one ray whose walk pushes far deeper than one stack entry per live lane
+    # (the wave's stack keeps entry i in lane i, so lanes without a ray must stay in the wave)
+    print("case: 65 camera rays, deep tree", flush=True)
+    sc = scenes.clustered_scene(20_000, 0, seed=23)
+    o, d = scenes.pinhole_rays(13, 5, (0.0, 0.0, -4.0))
+    a = _both(sc, o, d)
+    assert int(a["iters"].max()) > 0
+
+
 def test_start_inside_limits_and_sh():
     print("case: start inside, limits, SH", flush=True)
     sc = scenes.uniform_scene(300, 0, seed=4)
