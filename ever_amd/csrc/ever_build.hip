@@ -444,8 +444,9 @@ __device__ __forceinline__ Box range_union(int32_t lo, int32_t hi, const float* 
   return x;
 }
 
-// 4-wide node records: the record of binary node i describes up to four children (its children, with an internal
-// child replaced by that child's two children), boxes stored per axis (SoA) so one record is 128 B:
+// 4-wide node records: the record of binary node i describes up to four nodes of its binary subtree (its
+// children, then the internal slot with the largest box surface opened twice), boxes stored per axis (SoA) so one
+// record is 128 B:
 //   f4[0] = lo.x of children 0..3, f4[1] = hi.x, f4[2] = lo.y, f4[3] = hi.y, f4[4] = lo.z, f4[5] = hi.z,
 //   f4[6] = child refs (internal: binary node index of the child's record; leaf: ~primitive; empty: kEmptyChild)
 // Only records reachable from the root (binary node 0) are visited.
@@ -476,44 +477,59 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const float4 e = nodes2[i];
-  // the (up to) four grandchildren in order: a leaf child stands for itself, an internal child for its two
-  // children; the slots are filled with selects (no dynamically indexed arrays, which would live in scratch)
-  const int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-  int32_t ra0, ra1 = kEmptyChild, rb0, rb1 = kEmptyChild;
-  Box ba0, ba1 = box_empty(), bb0, bb1 = box_empty();
-  if (c0 < 0) {
-    ra0 = ~(int32_t)fprim[order[~c0]];
-    ba0 = box_load(sbox + 6 * (int64_t)(~c0));
-  } else {
-    const float4 ge = nodes2[c0];
-    const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
-    ba0 = child_box(g0, sbox, ibox);
-    ra0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
-    ba1 = child_box(g1, sbox, ibox);
-    ra1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
+  // Children of the 4-wide record: start from the node's two children and open, twice, the internal slot with the
+  // largest box surface (it is replaced by its two children), so a record has four children whenever its binary
+  // subtree has four nodes to offer. Slots are scalars filled with selects (dynamically indexed arrays would live
+  // in scratch); a slot holds the binary child encoding (internal >= 0, leaf ~index) or kSlotNone.
+  constexpr int32_t kSlotNone = INT32_MIN;
+  auto area = [](const Box& b) {
+    const float dx = b.hx - b.lx, dy = b.hy - b.ly, dz = b.hz - b.lz;
+    return dx * dy + dy * dz + dz * dx;
+  };
+  int32_t r0 = __float_as_int(e.x), r1 = __float_as_int(e.y), r2 = kSlotNone, r3 = kSlotNone;
+  Box b0 = child_box(r0, sbox, ibox), b1 = child_box(r1, sbox, ibox), b2 = box_empty(), b3 = box_empty();
+  {
+    const float a0 = r0 >= 0 ? area(b0) : -1.f, a1 = r1 >= 0 ? area(b1) : -1.f;
+    if (a0 >= 0.f || a1 >= 0.f) {
+      const bool o0 = a0 >= a1;
+      const float4 ge = nodes2[o0 ? r0 : r1];
+      const int32_t y = __float_as_int(ge.x), z = __float_as_int(ge.y);
+      const Box by = child_box(y, sbox, ibox), bz = child_box(z, sbox, ibox);
+      if (o0) {
+        r0 = y;
+        b0 = by;
+      } else {
+        r1 = y;
+        b1 = by;
+      }
+      r2 = z;
+      b2 = bz;
+      const float e0 = r0 >= 0 ? area(b0) : -1.f, e1 = r1 >= 0 ? area(b1) : -1.f, e2 = r2 >= 0 ? area(b2) : -1.f;
+      if (e0 >= 0.f || e1 >= 0.f || e2 >= 0.f) {
+        const int k = (e0 >= e1 && e0 >= e2) ? 0 : (e1 >= e2 ? 1 : 2);
+        const float4 ge2 = nodes2[k == 0 ? r0 : (k == 1 ? r1 : r2)];
+        const int32_t u = __float_as_int(ge2.x), v = __float_as_int(ge2.y);
+        const Box bu = child_box(u, sbox, ibox), bv = child_box(v, sbox, ibox);
+        if (k == 0) {
+          r0 = u;
+          b0 = bu;
+        } else if (k == 1) {
+          r1 = u;
+          b1 = bu;
+        } else {
+          r2 = u;
+          b2 = bu;
+        }
+        r3 = v;
+        b3 = bv;
+      }
+    }
   }
-  if (c1 < 0) {
-    rb0 = ~(int32_t)fprim[order[~c1]];
-    bb0 = box_load(sbox + 6 * (int64_t)(~c1));
-  } else {
-    const float4 ge = nodes2[c1];
-    const int32_t g0 = __float_as_int(ge.x), g1 = __float_as_int(ge.y);
-    bb0 = child_box(g0, sbox, ibox);
-    rb0 = g0 < 0 ? ~(int32_t)fprim[order[~g0]] : g0;
-    bb1 = child_box(g1, sbox, ibox);
-    rb1 = g1 < 0 ? ~(int32_t)fprim[order[~g1]] : g1;
-  }
-  const bool two = c0 >= 0;  // child 0 took two slots
-  int32_t ref[4];
-  Box bx[4];
-  ref[0] = ra0;
-  bx[0] = ba0;
-  ref[1] = two ? ra1 : rb0;
-  bx[1] = two ? ba1 : bb0;
-  ref[2] = two ? rb0 : rb1;
-  bx[2] = two ? bb0 : bb1;
-  ref[3] = two ? rb1 : kEmptyChild;
-  bx[3] = two ? bb1 : box_empty();
+  auto ref_of = [&](int32_t r) {
+    return r == kSlotNone ? kEmptyChild : (r < 0 ? ~(int32_t)fprim[order[~r]] : r);
+  };
+  const int32_t ref[4] = {ref_of(r0), ref_of(r1), ref_of(r2), ref_of(r3)};
+  const Box bx[4] = {b0, b1, b2, b3};
   float4* nd = nodes4 + 8 * i;
   nd[0] = make_float4(bx[0].lx, bx[1].lx, bx[2].lx, bx[3].lx);
   nd[1] = make_float4(bx[0].hx, bx[1].hx, bx[2].hx, bx[3].hx);
