@@ -168,6 +168,20 @@ def proposal_form(a, params, og, dg, grad, eye, nr, plain_value):
     e1.record()
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1) / 5
+    # rendering (forward only, no event list, no touch counts): plain walk vs network + proposal-bounded walk
+    def render(use_prop):
+        p_ = net.proposal(og, dg) if use_prop else None
+        return T.trace_forward(scene, og, dg, 0.0, 100.0, a.max_iters, store_events=False, count_touches=False,
+                               proposal=p_)["color"]
+    render_ms = {}
+    for key, use_prop in (("plain", False), ("proposal", True)):
+        render(use_prop)
+        e0.record()
+        for _ in range(5):
+            render(use_prop)
+        e1.record()
+        torch.cuda.synchronize()
+        render_ms[key] = e0.elapsed_time(e1) / 5
     with torch.no_grad():
         plain = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters)
         bounded = fes.trace_rays_sh(*params, og, dg, eye, 3, 0.0, 100.0, max_iters=a.max_iters, proposal=prop)
@@ -181,6 +195,9 @@ def proposal_form(a, params, og, dg, grad, eye, nr, plain_value):
             "color_max_abs_diff_vs_plain": float((plain - bounded).abs().max()),
             "proposal_rel_err_median": round(float(rel.median()), 4),
             "proposal_short_frac": round(float((prop[sat] < ideal_depth[sat]).float().mean()), 4),
+            "render": {"plain_ms": round(render_ms["plain"], 4), "proposal_ms": round(render_ms["proposal"], 4),
+                       "speedup": round(render_ms["plain"] / render_ms["proposal"], 4),
+                       "note": "forward only (no event list), network inference included"},
             "mlp": {"inference_ms": round(infer_ms, 4), "flops_per_ray": MLP_FLOPS_PER_RAY,
                     "achieved_tflops": round(achieved, 2), "peak_tflops": BF16_PEAK_TFLOPS,
                     "mfma_util": round(achieved / BF16_PEAK_TFLOPS, 5), "dtype": "bf16 x bf16 -> f32"},
