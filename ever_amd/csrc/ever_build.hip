@@ -1,5 +1,5 @@
 // LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (colour, traversal record,
-// centroid bounds), leaf fragments (elongated ellipsoids split along their major axis), 30-bit Morton codes, an LDS radix sort written for wave64, Karras' parallel hierarchy
+// centroid bounds), leaf fragments (elongated ellipsoids split along their major axis), Morton codes with a box size class, an LDS radix sort written for wave64, Karras' parallel hierarchy
 // construction, and node boxes computed as unions over each node's contiguous leaf range (in-block
 // prefix/suffix unions + a sparse table over 256-leaf blocks), which needs no atomics or cross-XCD coherence.
 //
@@ -135,8 +135,9 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
 // F = clamp(ceil(s_max / s_mid), 1, kMaxFragments) slabs; fragment j's box is the intersection of the
 // ellipsoid's tight AABB (create_aabbs.cu:77-114: half-extent_k = sqrt(sum_l (R_kl s_l)^2), q normalized) and
 // the AABB of the local box [a_j, b_j] x [-rho_j, rho_j]^2 that contains the slab (rho_j = sqrt(1 - min a^2)).
-// Boxes are padded by a relative 1e-6 so they stay conservative under rounding. Each fragment gets the 30-bit
-// Morton code of its box centre; unused slots get key 0xffffffff. The leaf count L is accumulated in header[9].
+// Boxes are padded by a relative 1e-6 so they stay conservative under rounding. Each fragment gets the 27-bit
+// Morton code (9 bits per axis) of its box centre with a 3-bit box size class inserted after its first 15 bits;
+// unused slots get key 0xffffffff. The leaf count L is accumulated in header[9].
 __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* __restrict__ header,
                                                         float* __restrict__ aabb, uint32_t* __restrict__ fprim,
                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* 
       hl[ax] = 0.5f * (b - a);
       float* bx = aabb + 6 * slot;
       uint32_t qk[3];
+      float maxext = 0.f, sext = 0.f;
       for (int k = 0; k < 3; ++k) {
         float cc = mv[k], e = 0.f;
         for (int l = 0; l < 3; ++l) {
@@ -193,9 +195,19 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* 
         bx[3 + k] = hi_k + pad;
         const float ext = hi[k] - lo[k];
         const float u = ext > 0.f ? (0.5f * (lo_k + hi_k) - lo[k]) / ext : 0.5f;
-        qk[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+        qk[k] = (uint32_t)fminf(fmaxf(u * 512.f, 0.f), 511.f);
+        maxext = fmaxf(maxext, hi_k - lo_k);
+        sext = fmaxf(sext, ext);
       }
-      keys[slot] = (expand_bits10(qk[0]) << 2) | (expand_bits10(qk[1]) << 1) | expand_bits10(qk[2]);
+      // size class of the box (0: the largest, an eighth of the scene or more; one class per halving, 7: the
+      // smallest), inserted into the Morton code after its first 15 bits: boxes of one region are grouped by size
+      // before they are split further, so large boxes end up high in the tree instead of widening the small-box
+      // nodes they would otherwise share (measured: forward 9.3 -> 8.1 ms at 1M/1080p; at the top of the code or
+      // after 9 bits it is worse, after 12, 18 or 21 bits or with 2 or 4 size bits no better)
+      const float rel = sext > 0.f && maxext > 0.f ? sext / maxext : 256.f;
+      const uint32_t sc = (uint32_t)fminf(fmaxf(floorf(log2f(rel)) - 1.f, 0.f), 7.f);
+      const uint32_t m27 = (expand_bits10(qk[0]) << 2) | (expand_bits10(qk[1]) << 1) | expand_bits10(qk[2]);
+      keys[slot] = ((m27 >> 12) << 15) | (sc << 12) | (m27 & 0xfffu);
       fprim[slot] = (uint32_t)i;
     }
   }
