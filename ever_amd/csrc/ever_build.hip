@@ -1,7 +1,7 @@
 // LBVH build over ellipsoid AABBs, rebuilt on every call: per-primitive prepass (colour, traversal record,
 // centroid bounds), leaf fragments (elongated ellipsoids split along their major axis), Morton codes with a box size class, an LDS radix sort written for wave64, Karras' parallel hierarchy
 // construction, and node boxes computed as unions over each node's contiguous leaf range (in-block
-// prefix/suffix unions + a sparse table over 256-leaf blocks), which needs no atomics or cross-XCD coherence.
+// prefix/suffix unions + a sparse table over 64-leaf blocks), which needs no atomics or cross-XCD coherence.
 //
 // Replaces the reference's OptiX acceleration structure (splinetracers/fast_ellipsoid_splinetracer/GAS.cpp:56-153)
 // and its AABB kernel (create_aabbs.cu:69-115), plus the per-primitive SH prepass (splinetracers/slang/sh_kernel.slang:19-42)
@@ -38,7 +38,24 @@ __global__ void k_build_init(uint32_t* header) {
 //   row 3 = (density, R, G, B).
 __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __restrict__ rec,
                                                       float4* __restrict__ colr, float* __restrict__ bpart) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // the block's SH features staged through LDS with contiguous loads (a lane reading its own primitive's
+  // coefficients would touch a different line per lane and load), 49 words per primitive so the lanes' reads
+  // fall in distinct banks
+  constexpr int kStride = 49;
+  __shared__ float fs[256 * kStride];
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  const int per = 3 * P.feature_size;
+  const bool staged = P.color_mode == EVER_COLOR_SH && per <= 48;
+  if (staged) {
+    const int64_t nb = P.num_prims - i0 < 256 ? P.num_prims - i0 : 256;
+    const float* src = P.features + per * i0;
+    for (int64_t idx = threadIdx.x; idx < nb * per; idx += blockDim.x) {
+      const int64_t pl = idx / per;
+      fs[pl * kStride + (idx - pl * per)] = src[idx];
+    }
+    __syncthreads();
+  }
+  int64_t i = i0 + threadIdx.x;
   bool valid = i < P.num_prims;
   float cx = 0.f, cy = 0.f, cz = 0.f;
   if (valid) {
@@ -48,7 +65,7 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
     cx = m.x; cy = m.y; cz = m.z;
     float dens = P.densities[i];
     float3 c;
-    const float* f = P.features + (int64_t)3 * P.feature_size * i;
+    const float* f = staged ? fs + threadIdx.x * kStride : P.features + (int64_t)3 * P.feature_size * i;
     if (P.color_mode == EVER_COLOR_SH) {
       auto colour_at = [&](f3 origin) {
         const f3 dir = sh_dir(m, origin);

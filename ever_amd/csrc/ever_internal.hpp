@@ -17,7 +17,7 @@ constexpr int kRadix = 256;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-constexpr int kLeafBlock = 256;   // leaves per block of the range-union refit
+constexpr int kLeafBlock = 64;    // leaves per block of the range-union refit
 constexpr int kMaxFragments = 4;
 constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 4-wide node  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
 
