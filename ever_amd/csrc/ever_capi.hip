@@ -190,6 +190,8 @@ int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspa
   if (int rc = check_prims(prims)) return rc;
   if (int rc = check_rays(rays)) return rc;
   if (!io) return fail(EVER_ERR_INVALID, "io is null");
+  if (rays->event_layout == EVER_EVENTS_BY_SLOT && rays->num_rays > 0 && !io->ray_order)
+    return fail(EVER_ERR_INVALID, "event lists stored by slot need the forward's ray_order");
   if (prims->num_views > 1 && prims->num_prims > 0 && rays->num_rays > 0 && (!rays->view_index || !io->view_color_grads))
     return fail(EVER_ERR_INVALID, "view_index and view_color_grads are needed when num_views > 1");
   if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");

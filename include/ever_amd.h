@@ -119,7 +119,7 @@ typedef struct ever_backward_io {
   const int32_t* tri_collection;
   const float* states;
   const double* ray_state64;
-  const int32_t* ray_order; /* forward's processing order; nullable (identity) */
+  const int32_t* ray_order; /* forward's processing order; nullable (identity) unless the events are stored by slot */
   const float* wcts; /* (W,4,4) world-to-camera matrices for the means2D statistic; nullable */
   int64_t num_wcts;
   float* dmeans;     /* (N,3) */

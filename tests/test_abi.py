@@ -74,6 +74,14 @@ def test_invalid_arguments_return_status_codes():
     assert L.ever_trace_forward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r), None,
                                 None) == 1
     assert b"event_layout" in L.ever_last_error()
+    # by-slot event lists cannot be read back without the forward's processing order
+    r.event_layout = C.EVER_EVENTS_BY_SLOT
+    r.num_rays = 1
+    r.origins = r.directions = 16  # never dereferenced: validation fails first
+    io = C.BackwardIO()
+    assert L.ever_trace_backward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r),
+                                 ctypes.byref(io), None) == 1
+    assert b"ray_order" in L.ever_last_error()
     assert L.ever_eval_sh(None, None, None, 5, 1, 4, None, None) == 1
 
 
