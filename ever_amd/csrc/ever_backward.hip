@@ -199,6 +199,12 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   int32_t id = tcol[(int64_t)(E - 1) * A.nr];
   int32_t id_next = E >= 2 ? tcol[(int64_t)(E - 2) * A.nr] : 0;   // event k - 1
   int32_t id_next2 = E >= 3 ? tcol[(int64_t)(E - 3) * A.nr] : 0;  // event k - 2
+  // contributions of consecutive events of one primitive (its exit and entry are often neighbours) are summed
+  // in registers and reach the block's table once: its float atomics in LDS are the adjoint's dominant cost
+  uint32_t pend_key = kEmptySlot;
+  float pend[kAccFields];
+#pragma unroll
+  for (int j = 0; j < kAccFields; ++j) pend[j] = 0.f;
   EventGeo G;
   double tk;
   {
@@ -291,7 +297,16 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     {
       const float v[kAccFields] = {-rob.x, -rob.y, -rob.z, sbar.x, sbar.y, sbar.z, qb.x, qb.y, qb.z, qb.w,
                                    densb, colb0, colb1, colb2};
-      scatter_prim(A, hkeys, hvals, A.vcol ? (uint32_t)(view * A.n + p) : (uint32_t)p, v);
+      const uint32_t key = A.vcol ? (uint32_t)(view * A.n + p) : (uint32_t)p;
+      if (key == pend_key) {
+#pragma unroll
+        for (int j = 0; j < kAccFields; ++j) pend[j] += v[j];
+      } else {
+        if (pend_key != kEmptySlot) scatter_prim(A, hkeys, hvals, pend_key, pend);
+        pend_key = key;
+#pragma unroll
+        for (int j = 0; j < kAccFields; ++j) pend[j] = v[j];
+      }
     }
     float* ac = A.acc + 16 * p;
     if (inv) {
@@ -317,6 +332,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     id_next = id_next2;
     id_next2 = id_next3;
   }
+  if (pend_key != kEmptySlot) scatter_prim(A, hkeys, hvals, pend_key, pend);
   *dinit = make_float4(Db0, Db1, Db2, Db3);
   float* o = A.drayo + 3 * r;
   float* dd = A.drayd + 3 * r;
