@@ -36,6 +36,9 @@ __global__ void k_build_init(uint32_t* header) {
 //   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
 //   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
 //   row 3 = (density, R, G, B).
+// kColorsOnly: only the colour part of the records (rec row 3, colr, per-view colours) is rewritten, for new
+// features on an unchanged geometry (ever_update_colors)
+template <bool kColorsOnly>
 __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __restrict__ rec,
                                                       float4* __restrict__ colr, float* __restrict__ bpart) {
   // the block's SH features staged through LDS with contiguous loads (a lane reading its own primitive's
@@ -96,6 +99,8 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
     }
     const float4 cr = make_float4(dens, c.x, c.y, c.z);
     colr[i] = cr;
+    rec[4 * i + 3] = cr;
+    if (kColorsOnly) return;
     const f3 c0 = qrot(mk3(1.f, 0.f, 0.f), q), c1 = qrot(mk3(0.f, 1.f, 0.f), q), c2 = qrot(mk3(0.f, 0.f, 1.f), q);
     const float M[3][3] = {{c0.x / s.x, c1.x / s.x, c2.x / s.x},
                            {c0.y / s.y, c1.y / s.y, c2.y / s.y},
@@ -104,8 +109,8 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
       const float bb = -(M[k][0] * m.x + M[k][1] * m.y + M[k][2] * m.z);
       rec[4 * i + k] = make_float4(M[k][0], M[k][1], M[k][2], bb);
     }
-    rec[4 * i + 3] = cr;
   }
+  if (kColorsOnly) return;
   // block reduction of centroid bounds into a per-block partial (reduced by k_bounds_final)
   __shared__ float red[6][4];
   float v[6] = {valid ? cx : INFINITY, valid ? cy : INFINITY, valid ? cz : INFINITY,
@@ -592,6 +597,15 @@ hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
   return hipGetLastError();
 }
 
+hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t st) {
+  const int64_t n = P.num_prims;
+  if (n <= 0) return hipSuccess;
+  SceneLayout L = scene_layout(n);
+  hipLaunchKernelGGL(k_prim_prepass<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, P, at<float4>(ws, L.rec),
+                     at<float4>(ws, L.colr), at<float>(ws, L.bpart));
+  return hipGetLastError();
+}
+
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   const int64_t n = P.num_prims;
   SceneLayout L = scene_layout(n);
@@ -617,7 +631,7 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, header);
   if (n <= 0) return hipGetLastError();
   const unsigned nb256 = grid_for(n, 256);
-  hipLaunchKernelGGL(k_prim_prepass, dim3(nb256), dim3(256), 0, st, P, rec, colr, bpart);
+  hipLaunchKernelGGL(k_prim_prepass<false>, dim3(nb256), dim3(256), 0, st, P, rec, colr, bpart);
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, bpart, (int64_t)nb256, header);
   hipLaunchKernelGGL(k_emit_fragments, dim3(nb256), dim3(256), 0, st, P, header, aabb, fprim, k0, v0);
   const int64_t S = L.slots;

@@ -165,6 +165,14 @@ int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_
   return hip_status(ever::launch_build(*prims, workspace, (hipStream_t)stream), "ever_build_scene");
 }
 
+int ever_update_colors(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream) {
+  if (int rc = check_prims(prims)) return rc;
+  if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
+  if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
+    return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
+  return hip_status(ever::launch_update_colors(*prims, workspace, (hipStream_t)stream), "ever_update_colors");
+}
+
 int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
                        const ever_forward_out* out, void* stream) {
   if (int rc = check_prims(prims)) return rc;

@@ -115,6 +115,8 @@ template <class T> inline const T* at(const void* base, size_t off) {
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
                              int bits, hipStream_t st);
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t s);
+// Rewrites the per-primitive colours of a built scene from new features (geometry and tree untouched).
+hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t s);
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
                           hipStream_t s);
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,

@@ -19,6 +19,7 @@ from torch.autograd import Function
 
 from .. import _capi as C
 from .. import tracer as T
+from . import sp  # noqa: F401  (the reference's compiled module of the same name, for its inference callers)
 
 # gradient clamps of the reference's backward (fast_ellipsoid_splinetracer.py:201-211) are applied in the
 # library's per-primitive epilogue and in the per-ray adjoint kernel.

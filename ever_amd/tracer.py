@@ -177,6 +177,25 @@ class Scene:
             _t1(tok)
         return self
 
+    def update_features(self, features):
+        """New colour features for the built scene (Forward.reset_features, Forward.cpp:252-255): only the
+        per-primitive colours are rewritten; the tree and geometry records stay as built."""
+        _check_float(features, "colors", 3, self.device)
+        if features.dim() != 3 or features.size(0) != self.n:
+            raise RuntimeError("All inputs (colors) must have the same 0 dimension")
+        means, scales, quats, densities, _ = self.tensors
+        self.tensors = (means, scales, quats, densities, features)
+        self.feature_size = features.size(1)
+        self.prims.features = features.data_ptr()
+        self.prims.feature_size = self.feature_size
+        L = C.lib()
+        with torch.cuda.device(self.device):
+            tok = _t0("update_colors")
+            C.check(L.ever_update_colors(ctypes.byref(self.prims), C.ptr(self.workspace), self.ws_bytes,
+                                         C.stream_ptr()), "ever_update_colors")
+            _t1(tok)
+        return self
+
 
 class SavedForBackward:
     """Mirror of fesSavedForBackward (py_binding.cpp:151-193)."""

@@ -11,6 +11,8 @@
  *   ever_build_scene      splinetracers/fast_ellipsoid_splinetracer/py_binding.cpp:99-132 (Primitives.add_primitives)
  *                         + create_aabbs.cu:117-132 + GAS.cpp:56-153 (GAS::build, OptiX BVH -> LBVH)
  *                         + splinetracers/slang/sh_kernel.slang:19-42 (per-primitive SH colour, color_mode 1)
+ *   ever_update_colors    py_binding.cpp:207-209 (Forward.update_model) -> Forward.cpp:252-255 (reset_features):
+ *                         new colour features for an already built scene, geometry and tree kept
  *   ever_trace_forward    py_binding.cpp:210-258 (Forward.trace_rays) -> Forward.cpp:23-82,
  *                         initialize_density.cu:152-208, slang/shaders.slang:171-235
  *   ever_trace_backward   splinetracers/fast_ellipsoid_splinetracer.py:105-219 (SplineTracer.backward) ->
@@ -148,6 +150,10 @@ int ever_scene_layout(int64_t num_prims, int64_t* offsets20);
 /* Scene workspace: LBVH nodes, sort buffers, per-primitive records. Lives from build to backward. */
 size_t ever_scene_workspace_bytes(int64_t num_prims);
 int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream);
+/* Rewrites only the per-primitive colours of a scene built from the same geometry (means, scales, quats,
+ * densities and num_prims unchanged; features, color_mode, SH origin/degree and view colours may differ):
+ * the tree, the primitive records' geometry rows and the bounds stay as built. */
+int ever_update_colors(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream);
 
 /* The scene workspace is also written: the start-inside candidate list of this ray batch. */
 size_t ever_forward_workspace_bytes(int64_t num_rays);

@@ -59,6 +59,9 @@ def test_invalid_arguments_return_status_codes():
     assert L.ever_build_scene(ctypes.byref(p), None, 0, None) == 1  # null workspace
     buf = ctypes.create_string_buffer(16)
     assert L.ever_build_scene(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 0, None) == 3
+    assert L.ever_update_colors(None, None, 0, None) == 1
+    assert L.ever_update_colors(ctypes.byref(p), None, 0, None) == 1  # null workspace
+    assert L.ever_update_colors(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 0, None) == 3
     r = C.Rays()
     r.kbuf = 7
     assert L.ever_trace_forward(ctypes.byref(p), ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(r), None,
@@ -90,3 +93,18 @@ def test_python_layer_validates_like_the_reference():
     from ever_amd import tracer as T
     with pytest.raises(RuntimeError, match="CUDA"):
         T.Scene(torch.zeros(2, 3), torch.zeros(2, 3), torch.zeros(2, 4), torch.zeros(2), torch.zeros(2, 1, 3))
+
+
+def test_sp_module_validates_like_the_reference():
+    # py_binding.cpp:99-138: tensors are checked before anything is built
+    import torch
+    from ever_amd.splinetracers.fast_ellipsoid_splinetracer import sp
+    sp.OptixContext("cpu")  # loads the library
+    prims = sp.Primitives("cpu")
+    z = lambda *s: torch.zeros(*s)
+    with pytest.raises(RuntimeError, match="means must be a CUDA tensor"):
+        prims.add_primitives(z(2, 3), z(2, 3), z(2, 4), z(2, 10).half(), z(2), z(2, 1, 3))
+    with pytest.raises(RuntimeError, match="call add_primitives first"):
+        sp.GAS(None, "cpu", prims, True, False, True)
+    with pytest.raises(RuntimeError, match="call add_primitives first"):
+        sp.Forward(None, "cpu", prims, False)
