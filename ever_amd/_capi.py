@@ -16,6 +16,9 @@ EVER_COLOR_DC = 0
 EVER_COLOR_SH = 1
 EVER_EVENTS_BY_RAY = 0
 EVER_EVENTS_BY_SLOT = 1
+EVER_CAMERA_PERSPECTIVE = 0
+EVER_CAMERA_FISHEYE = 1
+EVER_CAMERA_PANORAMIC = 2
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -27,6 +30,12 @@ class Prims(ctypes.Structure):
                 ("num_prims", _i64), ("feature_size", _i32), ("color_mode", _i32), ("sh_degree", _i32),
                 ("activation", _i32), ("sh_origin", ctypes.c_float * 3), ("num_views", _i32),
                 ("view_origins", _vp), ("view_colors", _vp)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", _i32), ("height", _i32), ("model", _i32), ("distorted", _i32), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("dist", ctypes.c_float * 6),
+                ("c2w", _vp)]
 
 
 class Rays(ctypes.Structure):
@@ -75,6 +84,7 @@ SIGNATURES = {
     "ever_backward_workspace_bytes": ([_i64, _i64, _i64], ctypes.c_size_t),
     "ever_trace_backward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
                              ctypes.POINTER(BackwardIO), _vp], ctypes.c_int),
+    "ever_camera_rays": ([ctypes.POINTER(Camera), _vp, _vp, _vp, _vp], ctypes.c_int),
     "ever_eval_sh": ([_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
     "ever_eval_sh_backward": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
 }

@@ -219,6 +219,23 @@ int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspa
   return hip_status(ever::launch_backward(*prims, workspace, *rays, *io, (hipStream_t)stream), "ever_trace_backward");
 }
 
+int ever_camera_rays(const ever_camera* camera, const float* jitter, float* origins, float* directions,
+                     void* stream) {
+  if (!camera) return fail(EVER_ERR_INVALID, "camera is null");
+  const ever_camera& C = *camera;
+  if (C.width < 0 || C.height < 0) return fail(EVER_ERR_INVALID, "width and height must be >= 0");
+  if ((int64_t)C.width * C.height > INT32_MAX) return fail(EVER_ERR_INVALID, "width * height must be < 2^31");
+  if (C.model != EVER_CAMERA_PERSPECTIVE && C.model != EVER_CAMERA_FISHEYE && C.model != EVER_CAMERA_PANORAMIC)
+    return fail(EVER_ERR_INVALID, "model must be EVER_CAMERA_PERSPECTIVE, _FISHEYE or _PANORAMIC");
+  if (C.distorted != 0 && C.distorted != 1) return fail(EVER_ERR_INVALID, "distorted must be 0 or 1");
+  if (!(C.fx != 0.f && C.fy != 0.f) || !std::isfinite(C.fx) || !std::isfinite(C.fy))
+    return fail(EVER_ERR_INVALID, "focal lengths must be finite and non-zero");
+  if ((int64_t)C.width * C.height > 0 && (!C.c2w || !origins || !directions))
+    return fail(EVER_ERR_INVALID, "null pointer");
+  return hip_status(ever::launch_camera_rays(C, jitter, origins, directions, (hipStream_t)stream),
+                    "ever_camera_rays");
+}
+
 int ever_eval_sh(const float* means, const float* features, const float* origin, int64_t num_prims,
                  int32_t feature_size, int32_t sh_degree, float* out_rgb, void* stream) {
   if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");

@@ -121,6 +121,8 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
                           hipStream_t s);
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
                            hipStream_t s);
+hipError_t launch_camera_rays(const ever_camera& C, const float* jitter, float* origins, float* directions,
+                              hipStream_t s);
 hipError_t launch_eval_sh(const float* means, const float* feat, const float* origin, int64_t n, int d, int deg,
                           float* out, hipStream_t s);
 hipError_t launch_eval_sh_bwd(const float* means, const float* feat, const float* origin, const float* drgb,

@@ -20,6 +20,9 @@
  *                         (backwards_initial_drgb_kernel), gradient clamps fast_ellipsoid_splinetracer.py:201-211
  *   ever_eval_sh          eval_sh.py:31-55 (EvalSH.forward) -> splinetracers/slang/sh_kernel.slang:19-42
  *   ever_eval_sh_backward eval_sh.py:57-68 (EvalSH.backward) -> splinetracers/slang/sh_kernel.slang:45-98
+ *   ever_camera_rays      integration_utils/gaussian_renderer/ever.py:29-121 (get_ray_directions, get_rays,
+ *                         camera2rays, camera2rays_full) -> integration_utils/utils/camera_utils_zipnerf.py:816-1040
+ *                         (pixels_to_rays: radial/tangential undistortion, fisheye, panoramic)
  */
 #ifndef EVER_AMD_H
 #define EVER_AMD_H
@@ -202,6 +205,26 @@ int ever_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, flo
 int ever_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
                    int64_t M, int64_t N, int64_t K, const float* bias, int32_t relu, float* C, int32_t splits,
                    float* part, void* stream);
+
+/* Camera rays of one image, one ray per pixel in row-major pixel order (r = y * width + x). */
+#define EVER_CAMERA_PERSPECTIVE 0
+#define EVER_CAMERA_FISHEYE 1   /* equidistant: angle to the optical axis = |camera-plane point|, clipped at pi */
+#define EVER_CAMERA_PANORAMIC 2 /* camera-plane point (u, v) = (azimuth, polar angle) */
+
+typedef struct ever_camera {
+  int32_t width;
+  int32_t height;
+  int32_t model;     /* EVER_CAMERA_* */
+  int32_t distorted; /* 1: undo the radial (k1..k4) and tangential (p1, p2) distortion (10 Newton steps) */
+  float fx, fy;      /* focal lengths in pixels */
+  float cx, cy;      /* principal point in pixels; camera-plane point ((x + jx - cx) / fx, (y + jy - cy) / fy) */
+  float dist[6];     /* k1, k2, k3, k4, p1, p2 */
+  const float* c2w;  /* device, 12 floats: row-major 3x4 camera-to-world [R | t] (OpenCV axes: +z forward) */
+} ever_camera;
+
+/* origins (W*H,3) = t; directions (W*H,3) = unit R * camera direction. jitter: device (W*H,2) pixel offsets, or null
+ * for pixel centres (0.5, 0.5). */
+int ever_camera_rays(const ever_camera* camera, const float* jitter, float* origins, float* directions, void* stream);
 
 /* dL/dfeatures of ever_eval_sh (sh_kernel.slang:45-98); dfeatures is overwritten. */
 int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,

@@ -86,6 +86,19 @@ def test_invalid_arguments_return_status_codes():
                                  ctypes.byref(io), None) == 1
     assert b"ray_order" in L.ever_last_error()
     assert L.ever_eval_sh(None, None, None, 5, 1, 4, None, None) == 1
+    cam = C.Camera()
+    assert L.ever_camera_rays(None, None, None, None, None) == 1
+    cam.width, cam.height, cam.fx, cam.fy = 4, 3, 1.0, 1.0
+    cam.model = 3
+    assert L.ever_camera_rays(ctypes.byref(cam), None, None, None, None) == 1
+    assert b"model" in L.ever_last_error()
+    cam.model, cam.fx = C.EVER_CAMERA_FISHEYE, 0.0
+    assert L.ever_camera_rays(ctypes.byref(cam), None, None, None, None) == 1
+    assert b"focal" in L.ever_last_error()
+    cam.fx = 1.0
+    assert L.ever_camera_rays(ctypes.byref(cam), None, None, None, None) == 1  # null c2w / outputs
+    cam.width = 0
+    assert L.ever_camera_rays(ctypes.byref(cam), None, None, None, None) == 0  # empty image: nothing to do
 
 
 def test_python_layer_validates_like_the_reference():
@@ -108,3 +121,13 @@ def test_sp_module_validates_like_the_reference():
         sp.GAS(None, "cpu", prims, True, False, True)
     with pytest.raises(RuntimeError, match="call add_primitives first"):
         sp.Forward(None, "cpu", prims, False)
+
+
+def test_camera_module_validates():
+    import torch
+    from ever_amd import cameras as CAM
+    with pytest.raises(RuntimeError, match="CUDA"):
+        CAM.camera_rays(4, 4, 1.0, 1.0, torch.eye(4))
+    with pytest.raises(ValueError, match="projection"):
+        CAM.projection_code("orthographic")
+    assert CAM.projection_code(None) == 0 and CAM.projection_code("PANO") == 2
