@@ -19,6 +19,8 @@ EVER_EVENTS_BY_SLOT = 1
 EVER_CAMERA_PERSPECTIVE = 0
 EVER_CAMERA_FISHEYE = 1
 EVER_CAMERA_PANORAMIC = 2
+EVER_PLY_ACTIVATE_NONE = 0
+EVER_PLY_ACTIVATE_3DGS = 1
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -36,6 +38,11 @@ class Camera(ctypes.Structure):
     _fields_ = [("width", _i32), ("height", _i32), ("model", _i32), ("distorted", _i32), ("fx", ctypes.c_float),
                 ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("dist", ctypes.c_float * 6),
                 ("c2w", _vp)]
+
+
+class PlyLayout(ctypes.Structure):
+    _fields_ = [("num_columns", _i32), ("mean", _i32 * 3), ("scale", _i32 * 3), ("rot", _i32 * 4), ("f_dc", _i32 * 3),
+                ("f_rest0", _i32), ("num_rest", _i32), ("opacity", _i32), ("density", _i32), ("activation", _i32)]
 
 
 class Rays(ctypes.Structure):
@@ -84,6 +91,10 @@ SIGNATURES = {
     "ever_backward_workspace_bytes": ([_i64, _i64, _i64], ctypes.c_size_t),
     "ever_trace_backward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
                              ctypes.POINTER(BackwardIO), _vp], ctypes.c_int),
+    "ever_scene_from_records": ([_vp, _i64, ctypes.POINTER(PlyLayout), _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                                ctypes.c_int),
+    "ever_morton_workspace_bytes": ([_i64], ctypes.c_size_t),
+    "ever_morton_order": ([_vp, _i64, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "ever_camera_rays": ([ctypes.POINTER(Camera), _vp, _vp, _vp, _vp], ctypes.c_int),
     "ever_eval_sh": ([_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
     "ever_eval_sh_backward": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),

@@ -219,6 +219,50 @@ int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspa
   return hip_status(ever::launch_backward(*prims, workspace, *rays, *io, (hipStream_t)stream), "ever_trace_backward");
 }
 
+int ever_scene_from_records(const float* records, int64_t num_prims, const ever_ply_layout* layout,
+                            const int32_t* order, float* means, float* scales, float* quats, float* densities,
+                            float* features, void* stream) {
+  if (!layout) return fail(EVER_ERR_INVALID, "layout is null");
+  const ever_ply_layout& L = *layout;
+  if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (L.num_columns < 1) return fail(EVER_ERR_INVALID, "num_columns must be >= 1");
+  if (L.num_rest != 0 && L.num_rest != 3 && L.num_rest != 8 && L.num_rest != 15)
+    return fail(EVER_ERR_INVALID, "num_rest must be 0, 3, 8 or 15");
+  if (L.activation != EVER_PLY_ACTIVATE_NONE && L.activation != EVER_PLY_ACTIVATE_3DGS)
+    return fail(EVER_ERR_INVALID, "activation must be EVER_PLY_ACTIVATE_NONE or _3DGS");
+  auto col_ok = [&](int32_t c) { return c >= 0 && c < L.num_columns; };
+  for (int a = 0; a < 3; ++a)
+    if (!col_ok(L.mean[a]) || !col_ok(L.scale[a]) || !col_ok(L.f_dc[a]))
+      return fail(EVER_ERR_INVALID, "a mean, scale or f_dc column is out of range");
+  for (int a = 0; a < 4; ++a)
+    if (!col_ok(L.rot[a])) return fail(EVER_ERR_INVALID, "a rot column is out of range");
+  if (L.num_rest > 0 && (!col_ok(L.f_rest0) || !col_ok(L.f_rest0 + 3 * L.num_rest - 1)))
+    return fail(EVER_ERR_INVALID, "the f_rest columns are out of range");
+  if (!col_ok(L.activation == EVER_PLY_ACTIVATE_3DGS ? L.opacity : L.density))
+    return fail(EVER_ERR_INVALID, "the opacity/density column is out of range");
+  if (num_prims > 0 && (!records || !means || !scales || !quats || !densities || !features))
+    return fail(EVER_ERR_INVALID, "null pointer");
+  if (num_prims > INT32_MAX) return fail(EVER_ERR_INVALID, "num_prims must be < 2^31");
+  return hip_status(ever::launch_scene_from_records(records, num_prims, L, order, means, scales, quats, densities,
+                                                    features, (hipStream_t)stream),
+                    "ever_scene_from_records");
+}
+
+size_t ever_morton_workspace_bytes(int64_t num_prims) {
+  return num_prims < 0 ? 0 : ever::morton_layout(num_prims).total;
+}
+
+int ever_morton_order(const float* means, int64_t num_prims, int32_t* order, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (num_prims > INT32_MAX) return fail(EVER_ERR_INVALID, "num_prims must be < 2^31");
+  if (num_prims > 0 && (!means || !order || !workspace)) return fail(EVER_ERR_INVALID, "null pointer");
+  if (workspace_bytes < ever_morton_workspace_bytes(num_prims))
+    return fail(EVER_ERR_WORKSPACE, "morton workspace too small");
+  return hip_status(ever::launch_morton_order(means, num_prims, order, workspace, (hipStream_t)stream),
+                    "ever_morton_order");
+}
+
 int ever_camera_rays(const ever_camera* camera, const float* jitter, float* origins, float* directions,
                      void* stream) {
   if (!camera) return fail(EVER_ERR_INVALID, "camera is null");

@@ -121,6 +121,14 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
                           hipStream_t s);
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
                            hipStream_t s);
+struct MortonLayout {
+  size_t box, part, keys0, vals0, keys1, vals1, hi, hist, total;
+};
+MortonLayout morton_layout(int64_t n);
+hipError_t launch_scene_from_records(const float* records, int64_t n, const ever_ply_layout& L, const int32_t* order,
+                                     float* means, float* scales, float* quats, float* densities, float* features,
+                                     hipStream_t s);
+hipError_t launch_morton_order(const float* means, int64_t n, int32_t* order, void* ws, hipStream_t s);
 hipError_t launch_camera_rays(const ever_camera& C, const float* jitter, float* origins, float* directions,
                               hipStream_t s);
 hipError_t launch_eval_sh(const float* means, const float* feat, const float* origin, int64_t n, int d, int deg,

@@ -20,6 +20,9 @@
  *                         (backwards_initial_drgb_kernel), gradient clamps fast_ellipsoid_splinetracer.py:201-211
  *   ever_eval_sh          eval_sh.py:31-55 (EvalSH.forward) -> splinetracers/slang/sh_kernel.slang:19-42
  *   ever_eval_sh_backward eval_sh.py:57-68 (EvalSH.backward) -> splinetracers/slang/sh_kernel.slang:45-98
+ *   ever_scene_from_records  splinetracers/fast_ellipsoid_splinetracer/ply_file_loader.cc:150-293 (ReadSceneFromFile,
+ *                         NormalizeSceneData) and the layout of utils/math_util.py:152-202 (save_ply)
+ *   ever_morton_order     ply_file_loader.cc:72-148 (SortComponentsToMortonOrder)
  *   ever_camera_rays      integration_utils/gaussian_renderer/ever.py:29-121 (get_ray_directions, get_rays,
  *                         camera2rays, camera2rays_full) -> integration_utils/utils/camera_utils_zipnerf.py:816-1040
  *                         (pixels_to_rays: radial/tangential undistortion, fisheye, panoramic)
@@ -225,6 +228,40 @@ typedef struct ever_camera {
 /* origins (W*H,3) = t; directions (W*H,3) = unit R * camera direction. jitter: device (W*H,2) pixel offsets, or null
  * for pixel centres (0.5, 0.5). */
 int ever_camera_rays(const ever_camera* camera, const float* jitter, float* origins, float* directions, void* stream);
+
+/* Primitive arrays from PLY records: records is a device (N, num_columns) float32 array, one row per primitive
+ * (the file's vertex payload); the layout names the column of each field. f_dc: 3 columns; f_rest: 3 * num_rest
+ * consecutive columns from f_rest0, channel-major (f_rest[c * num_rest + k], as 3D Gaussian Splatting files and
+ * save_ply store them). Outputs (N,3) means, (N,3) scales, (N,4) quats, (N) densities, (N, 1 + num_rest, 3)
+ * features. EVER_PLY_ACTIVATE_NONE copies stored values (density column); EVER_PLY_ACTIVATE_3DGS applies the
+ * viewer's activations to a 3DGS file (opacity column): scales min(softplus(s) + 3e-4, 25), density
+ * max(0, -log(max(1 - 0.99 sigmoid(opacity), 1e-10))) / min(2 s), quaternions divided by their squared length.
+ * order: (N) source row of each output row (e.g. ever_morton_order), nullable for identity. */
+#define EVER_PLY_ACTIVATE_NONE 0
+#define EVER_PLY_ACTIVATE_3DGS 1
+
+typedef struct ever_ply_layout {
+  int32_t num_columns;
+  int32_t mean[3];
+  int32_t scale[3];
+  int32_t rot[4];
+  int32_t f_dc[3];
+  int32_t f_rest0;
+  int32_t num_rest; /* SH coefficients per channel beyond the DC one: 0, 3, 8 or 15 */
+  int32_t opacity;  /* EVER_PLY_ACTIVATE_3DGS */
+  int32_t density;  /* EVER_PLY_ACTIVATE_NONE */
+  int32_t activation;
+} ever_ply_layout;
+
+int ever_scene_from_records(const float* records, int64_t num_prims, const ever_ply_layout* layout,
+                            const int32_t* order, float* means, float* scales, float* quats, float* densities,
+                            float* features, void* stream);
+
+/* Morton order of the primitives: order[j] = source index of the j-th primitive sorted by the 63-bit Morton code
+ * of its mean (21 bits per axis over the means' bounding box); ties keep index order. */
+size_t ever_morton_workspace_bytes(int64_t num_prims);
+int ever_morton_order(const float* means, int64_t num_prims, int32_t* order, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 /* dL/dfeatures of ever_eval_sh (sh_kernel.slang:45-98); dfeatures is overwritten. */
 int ever_eval_sh_backward(const float* means, const float* features, const float* origin, const float* drgb,

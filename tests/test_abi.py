@@ -86,6 +86,18 @@ def test_invalid_arguments_return_status_codes():
                                  ctypes.byref(io), None) == 1
     assert b"ray_order" in L.ever_last_error()
     assert L.ever_eval_sh(None, None, None, 5, 1, 4, None, None) == 1
+    lay = C.PlyLayout()
+    assert L.ever_scene_from_records(None, 1, None, None, None, None, None, None, None, None) == 1
+    lay.num_columns, lay.num_rest = 10, 4
+    assert L.ever_scene_from_records(None, 1, ctypes.byref(lay), None, None, None, None, None, None, None) == 1
+    assert b"num_rest" in L.ever_last_error()
+    lay.num_rest, lay.f_dc[0] = 0, 12
+    assert L.ever_scene_from_records(None, 1, ctypes.byref(lay), None, None, None, None, None, None, None) == 1
+    assert b"out of range" in L.ever_last_error()
+    assert L.ever_morton_order(None, 5, None, None, 0, None) == 1
+    assert L.ever_morton_order(ctypes.cast(buf, ctypes.c_void_p), 5, ctypes.cast(buf, ctypes.c_void_p),
+                               ctypes.cast(buf, ctypes.c_void_p), 16, None) == 3
+    assert L.ever_morton_workspace_bytes(1000) > L.ever_morton_workspace_bytes(10) > 0
     cam = C.Camera()
     assert L.ever_camera_rays(None, None, None, None, None) == 1
     cam.width, cam.height, cam.fx, cam.fy = 4, 3, 1.0, 1.0
