@@ -177,9 +177,10 @@ class Scene:
             _t1(tok)
         return self
 
-    def update_features(self, features):
+    def update_features(self, features, sh_origin=None, sh_degree=None):
         """New colour features for the built scene (Forward.reset_features, Forward.cpp:252-255): only the
-        per-primitive colours are rewritten; the tree and geometry records stay as built."""
+        per-primitive colours are rewritten; the tree and geometry records stay as built. With SH colours the
+        origin and degree of the SH evaluation may change too (a new camera for the same scene)."""
         _check_float(features, "colors", 3, self.device)
         if features.dim() != 3 or features.size(0) != self.n:
             raise RuntimeError("All inputs (colors) must have the same 0 dimension")
@@ -188,6 +189,12 @@ class Scene:
         self.feature_size = features.size(1)
         self.prims.features = features.data_ptr()
         self.prims.feature_size = self.feature_size
+        if sh_origin is not None:
+            self.sh_origin = [float(v) for v in sh_origin]
+            self.prims.sh_origin = (ctypes.c_float * 3)(*self.sh_origin)
+        if sh_degree is not None:
+            self.sh_degree = int(sh_degree)
+            self.prims.sh_degree = self.sh_degree
         L = C.lib()
         with torch.cuda.device(self.device):
             tok = _t0("update_colors")

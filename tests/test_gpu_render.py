@@ -93,3 +93,33 @@ def test_splinerender_equals_the_reference_composition(model, dist):
         rel = float((x.grad - y.grad).norm() / max(float(y.grad.norm()), 1e-12))
         print(name, rel, flush=True)
         assert rel < 1e-4, name
+
+
+def test_fast_renderer_equals_the_sp_composition():
+    # the reference's FastRenderer frame (fast_renderer.py:110-131) through the sp module with tensor-computed
+    # colours, against the fused renderer, over two views of one model (the tree is built once)
+    from ever_amd.render import FastRenderer
+    from ever_amd.splinetracers.fast_ellipsoid_splinetracer import sp
+    print("case: FastRenderer, two views", flush=True)
+    sc = scenes.uniform_scene(5000, 3, seed=33)
+    pc = _Model(sc, 3)
+    views = [_view((0.3, 0.4, -3.5), 96, 72), _view((-2.0, 1.0, -2.5), 80, 60)]
+    fr = FastRenderer(views[0], pc, False)
+    with torch.no_grad():
+        prims = sp.Primitives(D)
+        half = torch.cat([pc._xyz, pc.scales, pc.rot], 1).half().contiguous()
+        prims.add_primitives(pc._xyz.detach(), pc.scales.detach(), pc.rot.detach(), half, pc.density.detach(),
+                             fr.get_color(views[0]))
+        ctx = sp.OptixContext(D)
+        gas = sp.GAS(ctx, D, prims, True, False, True)
+        fwd = sp.Forward(ctx, D, prims, False)
+        for view in views:
+            img = fr.render(view, pc, None)
+            prims.set_features(fr.get_color(view))
+            fwd.update_model(prims)
+            o, d = CAM.view_rays(view, device=D)
+            ref = fwd.trace_rays(gas, o, d, 0.0, 1e7, 200, 1000)["color"]
+            ref_img = ref[:, :3].T.reshape(3, view.image_height, view.image_width)
+            err = float((img - ref_img).abs().max())
+            print("max abs diff", err, flush=True)
+            assert img.shape == (3, view.image_height, view.image_width) and err < 1e-5
