@@ -224,6 +224,9 @@ struct RayState {
   uint64_t cursor;
   int iter;
   float bound;  // events beyond this depth are left for a later (plain) traversal; +inf once plain
+  // depth of the last event of the last completed 16-event round of the reference (the start of its current
+  // trace): an exit beyond tmax is gathered only when its entry lies after it (see leaf_events)
+  float tref;
 };
 
 // A lane of the per-wave walk with no ray: it stays in the wave (the wave's stack keeps entry i in lane i, and a
@@ -241,6 +244,7 @@ __device__ __forceinline__ void ray_idle(const FwdArgs& A, RayState& S) {
   S.cursor = 0;
   S.iter = A.max_iters;
   S.bound = INFINITY;
+  S.tref = 0.f;
 }
 
 template <bool kViews>
@@ -296,6 +300,7 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
   S.cursor = pack_event(0.f, 0xffffffffu);
   S.iter = 0;
   S.bound = INFINITY;
+  S.tref = 0.f;
 }
 
 // the ray's proposal, if any (NaN or negative proposals are ignored)
@@ -369,7 +374,10 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
   const bool hit = h >= 0.f && !(t1 < S.cur_t);
   const bool with_entry = t0 >= S.cur_t && ve > S.cursor;
   const bool ins_e = hit && with_entry && t0 <= A.tmax;
-  const bool ins_x = hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && t1 <= A.tmax));
+  // an exit beyond tmax enters only with its entry: in the reference's trace of the current round (from tref) the
+  // entry is reported with the exit as its attribute; an entry before tref was reported in an earlier round
+  const bool ins_x =
+      hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && (t1 <= A.tmax || (t0 > S.tref && t0 <= A.tmax))));
   if (kBounded) {
     // under a proposal only events up to it are gathered, so that every gathered event precedes every culled one
     e = ins_e && t0 <= S.bound ? ve : kNoEvent;
@@ -395,10 +403,16 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
     ddy *= nn;
     ddz *= nn;
   }
+  // Results equal the reference's 16-event rounds (BUFFER_SIZE, shaders.slang:18) for any K: events up to tmax
+  // do not depend on the rounds, and an exit beyond tmax is eligible when its entry lies after the start of the
+  // reference's current round (tref). Once this batch has crossed a round boundary, exits beyond tmax in it were
+  // gathered under the earlier tref, so the batch stops before them and the next traversal gathers again.
+  bool crossed = false;
   for (int k = 0; k < K; ++k) {
     const uint64_t head = head_at(k);
     const float tk = __uint_as_float((uint32_t)(head >> 32));
     if (tk > kEndT) return true;
+    if (crossed && tk > A.tmax) return false;
     const uint32_t key = (uint32_t)head;
     const uint32_t s = key >> 1;
     const bool entry = (key & 1u) == 0u;
@@ -435,6 +449,10 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
     if (A.tri) A.tri[S.e + (int64_t)S.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
     if (A.touch) atomicAdd(A.touch + orig, 1);
     ++S.iter;
+    if ((S.iter & 15) == 0) {
+      S.tref = tk;
+      crossed = true;
+    }
     if (!ray_alive(A, S)) return false;
   }
   return false;

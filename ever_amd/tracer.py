@@ -13,7 +13,8 @@ import torch
 from . import _capi as C
 
 BUFFER_SIZE = 16  # events gathered per traversal (shaders.slang:18)
-# k-buffer sizes with compiled kernels; results do not depend on the size (19 is the largest whose per-wave LDS
+# k-buffer sizes with compiled kernels; results do not depend on the size, they follow the reference's 16-event
+# rounds (which decide which exits beyond a cutting tmax are integrated) (19 is the largest whose per-wave LDS
 # buffer, 19 x 64 x 8 B plus a 256 B prefetch area, still lets 16 waves share a CU's 160 KB)
 KBUF_SIZES = (8, 16, 19, 32)
 _KBUF = int(os.environ.get("EVER_KBUF", BUFFER_SIZE))

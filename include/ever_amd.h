@@ -85,13 +85,15 @@ typedef struct ever_rays {
   float tmin;
   float tmax;
   int32_t max_iters;
-  int32_t kbuf; /* events gathered per traversal: 8, 16 (reference), 19 or 32; results do not depend on it */
+  /* events gathered per traversal: 8, 16 (reference), 19 or 32. Results do not depend on it: they follow the
+   * reference's 16-event rounds, which matter only for exits beyond a tmax that cuts through primitives (an
+   * exit beyond tmax is integrated when its entry lies in the same 16-event round, shaders.slang:96-159). */
+  int32_t kbuf;
   int32_t traversal; /* EVER_TRAVERSAL_*: who walks the tree, one lane per ray or the 64 rays of a wave */
   /* (R) proposed termination depth per ray, nullable. Depths are measured along the unit direction from
    * origin + tmin * direction. A ray first gathers only events up to its proposal (nodes beyond it are not
    * entered); if it has not terminated by then it continues with the plain walk, so results equal the plain
-   * trace's (an exit beyond tmax whose entry lies before the proposal is the one case the reference's own
-   * chunking rule treats differently). Not in the reference: the learned-proposal form. */
+   * trace's. Not in the reference: the learned-proposal form. */
   const float* proposal;
   const int32_t* view_index; /* (R) view of each ray when prims.num_views > 1, else ignored */
   /* Persistent launch of the per-wave walk (not in the reference): 0 = one wave per block over the whole grid;

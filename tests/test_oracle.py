@@ -375,3 +375,23 @@ def test_sh_basis_orthonormal():
             B[i0:i0 + 20000, k] = oracle.eval_sh(dd, np.repeat(f, len(dd), 0), 3, np.zeros(3))[:, 0] - 0.5
     G = (B * wq[:, None]).T @ B
     np.testing.assert_allclose(G, np.eye(16), atol=2e-4)
+
+
+def test_buffer_rounds_matter_at_a_cutting_tmax():
+    # The reference gathers 16 events per traversal (shaders.slang:18, 195-226); an exit beyond tmax is gathered
+    # only together with its entry, in the round whose trace reports that entry (:96-106 with OptiX's [tmin, tmax]
+    # report interval). So with a tmax that cuts through primitives, the buffer size changes which exits beyond
+    # tmax a ray integrates: the oracle, a literal restatement, shows it, and the HIP tracer must follow the
+    # 16-event rounds whatever its own buffer size (tests/test_gpu_sweep.py).
+    import test_gpu_sweep as S
+    from gpu_util import KEYS
+    c = S._draw(17)
+    sc = [c["sc"][k].double().numpy() for k in KEYS]
+    o, d = c["o"].double().numpy(), c["d"].double().numpy()
+    a = oracle.forward(*sc, o, d, c["tmin"], c["tmax"], max_iters=c["max_iters"], use_bvh=True, kbuf=16)
+    b = oracle.forward(*sc, o, d, c["tmin"], c["tmax"], max_iters=c["max_iters"], use_bvh=True, kbuf=19)
+    assert c["tmax"] < 10 and (a["iters"] != b["iters"]).sum() > 0
+    # without the cut the buffer size does not matter
+    a = oracle.forward(*sc, o, d, c["tmin"], 100.0, max_iters=c["max_iters"], use_bvh=True, kbuf=16)
+    b = oracle.forward(*sc, o, d, c["tmin"], 100.0, max_iters=c["max_iters"], use_bvh=True, kbuf=19)
+    assert np.array_equal(a["iters"], b["iters"]) and np.abs(a["color"] - b["color"]).max() < 1e-12
