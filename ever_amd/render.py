@@ -130,7 +130,7 @@ class FastRenderer:
                                        sh_degree=self.pc.active_sh_degree)
             out = T.trace_forward(self.scene, rayo.contiguous(), rayd.contiguous(), tmin, tmax, FAST_MAX_ITERS,
                                   store_events=False, count_touches=False)
-        return dict(color=out["color"][:, :4], iters=out["iters"], states=out["states"],
+        return dict(color=out["color"][:, :4].contiguous(), iters=out["iters"], states=out["states"],
                     initial_drgb=out["initial_drgb"])
 
     def render(self, view, pc, bg_color, tmin=None, scaling_modifier=1.0):
