@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=131072, help="rays traced by the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal form")
+    ap.add_argument("--no-inference", action="store_true", help="skip the inference-renderer frame timing")
     ap.add_argument("--mlp-epochs", type=int, default=12)
     return ap.parse_args()
 
@@ -210,6 +211,34 @@ def proposal_form(a, params, og, dg, grad, eye, nr, plain_value):
                          "margin": PROP_MARGIN}}
 
 
+def inference_frames(params, og, dg, eye, nr, frames=10):
+    """The inference renderer's frame (integration_utils/gaussian_renderer/fast_renderer.py:110-131 on this
+    library: ever_amd.render.FastRenderer) on the same scene and camera: the tree built once, then per frame the
+    colours rewritten for the camera (fused SH + softplus pass) and a forward trace without an event list, 200
+    events at most per ray as the reference's renderer asks."""
+    import torch
+
+    from ever_amd import tracer as T
+    with torch.no_grad():
+        t = [x.detach() for x in params]
+        scene = T.Scene(*t, color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
+
+        def frame():
+            scene.update_features(t[4], sh_origin=eye, sh_degree=3)
+            return T.trace_forward(scene, og, dg, 0.0, 1e7, 200, store_events=False, count_touches=False)
+
+        for _ in range(2):
+            frame()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            frame()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / frames
+    return {"ms_per_frame": round(ms, 3), "fps": round(1e3 / ms, 2), "Mrays_s": round(nr / ms / 1e3, 2),
+            "frames": frames, "note": "tree built once; per frame: fused recolouring + forward, no event list"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -296,6 +325,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": int(dom_bytes), "launch_ms": round(dom_ms, 4)},
     }
+    if not a.no_inference:
+        rec["inference"] = inference_frames(params, og, dg, eye, nr)
     if not a.no_proposal:
         rec["proposal"] = proposal_form(a, params, og, dg, grad, eye, nr, rec["value"])
     if not a.no_cpu and a.cpu_sample > 0:
