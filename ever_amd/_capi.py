@@ -54,7 +54,7 @@ class Rays(ctypes.Structure):
 class ForwardOut(ctypes.Structure):
     _fields_ = [("color", _vp), ("states", _vp), ("iters", _vp), ("tri_collection", _vp), ("touch_count", _vp),
                 ("initial_drgb", _vp), ("ray_state64", _vp), ("ray_order", _vp), ("workspace", _vp),
-                ("workspace_bytes", ctypes.c_size_t)]
+                ("workspace_bytes", ctypes.c_size_t), ("last_dirac", _vp)]
 
 
 class BackwardIO(ctypes.Structure):

@@ -47,6 +47,7 @@ struct FwdArgs {
   int32_t* touch;
   float* init_drgb;
   double* st64;
+  float* last_dirac;  // (R,4), nullable
   int32_t* spill;
   const uint32_t* order;  // processing order: thread slot -> ray index
   const float* proposal;  // per-ray proposed termination depth, nullable
@@ -491,6 +492,18 @@ __device__ __forceinline__ void ray_write(const FwdArgs& A, const RayState& S) {
   s64[3] = S.D3;
   s64[4] = S.logT;
   s64[5] = S.t_state;
+  if (A.last_dirac) {
+    // the last integrated event's control point (get_ctrl_pt, shaders.slang:65-82), recovered from the cursor
+    float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (S.iter > 0) {
+      const uint32_t key = (uint32_t)S.cursor;
+      const uint32_t s = key >> 1;
+      const float4 Dr = A.vcol ? A.vcol[(int64_t)A.view_index[r] * A.n + s] : A.rec[4 * (int64_t)s + 3];
+      const float h = (key & 1u) == 0u ? Dr.x : -Dr.x;
+      dv = make_float4(h, h * Dr.y, h * Dr.z, h * Dr.w);
+    }
+    reinterpret_cast<float4*>(A.last_dirac)[r] = dv;
+  }
 }
 
 // Per-ray traversal: one lane walks the tree for its own ray with a short LDS stack (global spill beyond
@@ -887,6 +900,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.touch = O.touch_count;
   A.init_drgb = O.initial_drgb;
   A.st64 = O.ray_state64;
+  A.last_dirac = O.last_dirac;
   A.spill = at<int32_t>(O.workspace, FL.spill);
   A.order = v0;
   A.proposal = R.proposal;

@@ -62,7 +62,7 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     layout = "ray" if return_extras else "slot"
     out = T.trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=not recompute,
                           zero_tail=return_extras, proposal=proposal, count_touches=return_extras,
-                          event_layout=layout)
+                          event_layout=layout, last_dirac=return_extras)
     ctx.event_layout = layout
     ctx.recompute = recompute
     ctx.proposal = proposal
@@ -80,7 +80,8 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
         nr = rayo.shape[0]
         saved = T.SavedForBackward(
             states=out["states"],
-            diracs=torch.zeros((nr, 4), dtype=torch.float32, device=ctx.device),
+            diracs=out["last_dirac"],
+            # last_tri, never updated by the reference (shaders.slang:187, 233)
             faces=torch.full((nr,), -1, dtype=torch.int32, device=ctx.device),
             touch_count=out["touch_count"],
             iters=out["iters"],

@@ -138,7 +138,7 @@ class Forward:
             nr = ray_origins.numel() // 3
             out = T.trace_forward(scene, ray_origins, ray_directions, tmin, tmax, max_iters,
                                   store_events=self.enable_backward, zero_tail=self.enable_backward,
-                                  count_touches=self.enable_backward, event_layout="ray")
+                                  count_touches=self.enable_backward, event_layout="ray", last_dirac=True)
             dev = scene.device
             tri = out["tri_collection"]
             touch = out["touch_count"]
@@ -151,8 +151,8 @@ class Forward:
                                torch.zeros((), dtype=torch.int32, device=dev))
             saved = SavedForBackward(
                 states=out["states"],
-                diracs=torch.zeros((nr, 4), dtype=torch.float32, device=dev),
-                faces=torch.full((nr,), -1, dtype=torch.int32, device=dev),
+                diracs=out["last_dirac"],
+                faces=torch.full((nr,), -1, dtype=torch.int32, device=dev),  # never updated (shaders.slang:187, 233)
                 touch_count=touch,
                 iters=out["iters"],
             )

@@ -217,14 +217,16 @@ class SavedForBackward:
 
 
 def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, zero_tail=False, kbuf=None,
-                  traversal=None, proposal=None, view_index=None, tile=0, count_touches=True, event_layout="ray"):
+                  traversal=None, proposal=None, view_index=None, tile=0, count_touches=True, event_layout="ray",
+                  last_dirac=False):
     """Forward trace (Forward.trace_rays, py_binding.cpp:210-258). Returns a dict of tensors.
 
     count_touches=False skips the per-primitive event counts (touch_count is then None): one global atomic per
     event that only the caller's extras ever read. event_layout "ray" stores event k of ray r at
     tri_collection[r + k*R] (the reference's layout); "slot" stores it at the ray's position in ray_order, so that
     the rays traced together write, and the backward reads, contiguous rows (trace_backward follows the forward's
-    choice)."""
+    choice). last_dirac=True also returns each ray's last integrated control point (the reference's
+    SavedForBackward.diracs)."""
     if event_layout not in EVENT_LAYOUTS:
         raise ValueError("event_layout must be 'ray' or 'slot'")
     _check_float(rayo, "ray_origins", 3, scene.device)
@@ -245,6 +247,7 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     init = torch.empty((nr, 4), dtype=torch.float32, device=dev)
     st64 = torch.empty((nr, 6), dtype=torch.float64, device=dev)
     order = torch.empty((nr,), dtype=torch.int32, device=dev)
+    dirac = torch.empty((nr, 4), dtype=torch.float32, device=dev) if last_dirac else None
     fws_bytes = L.ever_forward_workspace_bytes(nr)
     fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
     if proposal is not None:
@@ -258,14 +261,15 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
                   TRAVERSALS[traversal or _TRAVERSAL], C.ptr(proposal), C.ptr(view_index), int(tile),
                   EVENT_LAYOUTS[event_layout])
     out = C.ForwardOut(C.ptr(color), C.ptr(states), C.ptr(iters), C.ptr(tri), C.ptr(touch), C.ptr(init),
-                       C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes)
+                       C.ptr(st64), C.ptr(order), C.ptr(fws), fws_bytes, C.ptr(dirac))
     with torch.cuda.device(dev):
         tok = _t0("forward")
         C.check(L.ever_trace_forward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
                                      ctypes.byref(rays), ctypes.byref(out), C.stream_ptr()), "ever_trace_forward")
         _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
-                ray_state64=st64, ray_order=order, rays=rays, view_index=view_index, event_layout=event_layout)
+                ray_state64=st64, ray_order=order, rays=rays, view_index=view_index, event_layout=event_layout,
+                last_dirac=dirac)
 
 
 def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
@@ -301,7 +305,8 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
     if scene.num_views > 1:
         vgrad = torch.empty((scene.num_views, n, 4), dtype=torch.float32, device=dev)
     io = C.BackwardIO(C.ptr(g), C.ptr(fwd["iters"]), C.ptr(fwd["tri_collection"]), C.ptr(fwd["states"]),
-                      C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]), C.ptr(out["dscales"]),
+                      C.ptr(fwd["ray_state64"]), C.ptr(fwd.get("ray_order")), C.ptr(wcts), nw, C.ptr(out["dmeans"]),
+                      C.ptr(out["dscales"]),
                       C.ptr(out["dquats"]), C.ptr(out["ddensities"]), C.ptr(out["dfeatures"]), C.ptr(out["drayo"]),
                       C.ptr(out["drayd"]), C.ptr(out.get("dmeans2D")), C.ptr(out["dinitial_drgb"]), C.ptr(ws),
                       ws_bytes, C.ptr(vgrad))

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 5
+#define EVER_ABI_VERSION 6
 
 /* status codes */
 #define EVER_OK 0
@@ -121,6 +121,8 @@ typedef struct ever_forward_out {
   int32_t* ray_order;      /* (R) processing order (rays sorted for coherence), read by ever_trace_backward */
   void* workspace;         /* stack spill + ray sort, ever_forward_workspace_bytes(R) bytes */
   size_t workspace_bytes;
+  float* last_dirac; /* (R,4) control point of the last event integrated, (+-density, +-density * colour)
+                        (shaders.slang:229-232), zeros for a ray without events; nullable */
 } ever_forward_out;
 
 typedef struct ever_backward_io {

@@ -3,8 +3,9 @@
 Only tests/ import this module, as the checker of ever_amd.ply (ever_scene_from_records, ever_morton_order).
 Follows splinetracers/fast_ellipsoid_splinetracer/ply_file_loader.cc:150-193 (sigmoid, inv_opacity, softplus,
 NormalizeSceneData) with the loader's float32 roundings, and :72-148 (SortComponentsToMortonOrder), whose float32
-arithmetic is reproduced operation for operation so that codes are exact. Pinned in tests/test_ply_oracle.py by closed forms of
-the activations, by de-interleaving the codes back to the quantised coordinates, and by brute-force ordering.
+arithmetic is reproduced operation for operation so that codes are exact. Pinned in tests/test_ply_oracle.py by
+closed forms of the activations, by de-interleaving the codes back to the quantised coordinates, and by brute-force
+ordering.
 """
 import numpy as np
 

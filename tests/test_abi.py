@@ -28,7 +28,7 @@ def test_every_header_function_is_exported_and_bound():
 
 def test_abi_version_and_struct_sizes():
     L = C.lib()
-    assert L.ever_abi_version() == 5
+    assert L.ever_abi_version() == 6
     sizes = (ctypes.c_int64 * 4)()
     assert L.ever_struct_sizes(sizes) == 0
     assert list(sizes) == [ctypes.sizeof(C.Prims), ctypes.sizeof(C.Rays), ctypes.sizeof(C.ForwardOut),

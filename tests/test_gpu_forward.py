@@ -169,3 +169,27 @@ def test_events_by_slot_are_the_by_ray_events_permuted(traversal):
     gb = T.trace_backward(scene, og, dg, 0.0, 100.0, 500, b, g)
     for k in ga:  # per-primitive sums are float atomics: equal up to summation order
         assert float((ga[k] - gb[k]).norm()) <= 1e-6 * float(ga[k].norm()) + 1e-12, k
+
+
+@pytest.mark.parametrize("traversal", ["ray", "wave"])
+def test_last_dirac_is_the_last_integrated_control_point(traversal):
+    # SavedForBackward.diracs (shaders.slang:229-232): +-density * (1, SH_C0 * f0 + 0.5) of each ray's last
+    # integrated event (entry +, exit -), zeros for a ray without events
+    sc, o, d = scenes.config1()
+    D = torch.device("cuda:0")
+    t = [sc[k].to(D).contiguous() for k in KEYS]
+    scene = T.Scene(*t).build()
+    out = T.trace_forward(scene, o.to(D).contiguous(), d.to(D).contiguous(), 0.0, 100.0, 500, traversal=traversal,
+                          last_dirac=True)
+    it = out["iters"].cpu().numpy()
+    tri = out["tri_collection"].cpu().numpy().reshape(500, -1)
+    nr = o.shape[0]
+    dens = sc["densities"].numpy().astype(np.float64)
+    col = sc["features"][:, 0, :].numpy().astype(np.float64) * 0.28209479177387814 + 0.5
+    want = np.zeros((nr, 4))
+    for r in np.nonzero(it > 0)[0]:
+        ev = tri[it[r] - 1, r]
+        h = dens[ev >> 1] * (1.0 if ev & 1 else -1.0)
+        want[r] = [h, h * col[ev >> 1, 0], h * col[ev >> 1, 1], h * col[ev >> 1, 2]]
+    assert (it == 0).any() and (it > 0).any()
+    np.testing.assert_allclose(out["last_dirac"].cpu().numpy(), want, rtol=1e-6, atol=1e-6)
