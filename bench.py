@@ -242,7 +242,8 @@ def inference_frames(params, og, dg, eye, nr, frames=10):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the tracer shards by rays, see DESIGN.md",
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); several GPUs would each trace a share of "
+              "the rays (image tiles or whole cameras) against a replica of the scene",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -1,4 +1,4 @@
-"""Host-side driver of the HIP tracer: input validation, buffer allocation (PyTorch), C-ABI calls.
+"""Host side of the HIP tracer: input validation, buffer allocation (PyTorch), C-ABI calls.
 
 Mirrors the reference's pybind layer (splinetracers/fast_ellipsoid_splinetracer/py_binding.cpp):
 input checks :33-61 and :104-116, output allocation :218-235, saved-for-backward tensors :151-193.
