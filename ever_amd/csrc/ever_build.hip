@@ -50,10 +50,11 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
   const int per = 3 * P.feature_size;
   const bool staged = P.color_mode == EVER_COLOR_SH && per <= 48;
   if (staged) {
-    const int64_t nb = P.num_prims - i0 < 256 ? P.num_prims - i0 : 256;
+    // block-local indices in 32 bits (a 64-bit division per element costs more than the load)
+    const int nb = P.num_prims - i0 < 256 ? (int)(P.num_prims - i0) : 256;
     const float* src = P.features + per * i0;
-    for (int64_t idx = threadIdx.x; idx < nb * per; idx += blockDim.x) {
-      const int64_t pl = idx / per;
+    for (int idx = threadIdx.x; idx < nb * per; idx += blockDim.x) {
+      const int pl = idx / per;
       fs[pl * kStride + (idx - pl * per)] = src[idx];
     }
     __syncthreads();
