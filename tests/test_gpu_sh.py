@@ -37,18 +37,21 @@ def test_eval_sh_forward_and_vjp(deg):
     del dirs
 
 
-def test_fused_sh_trace_equals_composition():
+@pytest.mark.parametrize("deg, act", [(3, "softplus"), (2, "softplus"), (1, "none"), (0, "softplus"), (3, "none")])
+def test_fused_sh_trace_equals_composition(deg, act):
+    # active degree below the features' (d = 16): the higher coefficients are ignored, as eval_sh does
     sc = scenes.uniform_scene(3000, 3, seed=7)
     o, d = scenes.pinhole_rays(48, 48, (0.3, -0.2, -3.0), target=(0.0, 0.0, 0.0))
     eye = o[0]
     ps = [sc[k].to(D).requires_grad_(True) for k in ("means", "scales", "quats", "densities", "features")]
     g = scenes.upstream_grad(o.shape[0], seed=9).to(D)
-    out = fes.trace_rays_sh(*ps, o.to(D), d.to(D), tuple(eye.tolist()), 3, 0.0, 100.0)
+    out = fes.trace_rays_sh(*ps, o.to(D), d.to(D), tuple(eye.tolist()), deg, 0.0, 100.0, activation=act)
     (out * g).sum().backward()
     grads = [p.grad.clone() for p in ps]
     qs = [sc[k].to(D).requires_grad_(True) for k in ("means", "scales", "quats", "densities", "features")]
-    col = E.eval_sh(qs[0], qs[4], eye.to(D), 3)
-    col = torch.nn.functional.softplus(col, beta=10)
+    col = E.eval_sh(qs[0], qs[4], eye.to(D), deg)
+    if act == "softplus":
+        col = torch.nn.functional.softplus(col, beta=10)
     f0 = ((col - 0.5) / C0).reshape(-1, 1, 3)
     out2 = fes.trace_rays(qs[0], qs[1], qs[2], qs[3], f0, o.to(D), d.to(D), 0.0, 100.0)
     (out2 * g).sum().backward()
