@@ -71,7 +71,11 @@ __device__ __forceinline__ void add_global(const BwdArgs& A, uint32_t key, int j
 // FP32 one (measured: adjoint 3.1 against 4.5 ms at 1M/1080p), and the per-block sums gain precision.
 __device__ __forceinline__ void scatter_prim(const BwdArgs& A, uint32_t* hkeys, double (*hvals)[kHashSlots], uint32_t p,
                                              const float (&v)[kAccFields]) {
+  // find the slot first (the lanes of a wave take different numbers of probes), then add with all lanes together:
+  // one copy of the 14 atomics, issued once per flush by every flushing lane of the wave
   uint32_t h = (p * 2654435761u) >> 23;  // 9 bits
+  bool found = false;
+#pragma unroll 1
   for (int probe = 0; probe < 16; ++probe) {
     uint32_t k = __hip_atomic_load(hkeys + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == kEmptySlot) {
@@ -79,14 +83,18 @@ __device__ __forceinline__ void scatter_prim(const BwdArgs& A, uint32_t* hkeys, 
       k = (old == kEmptySlot) ? p : old;
     }
     if (k == p) {
-#pragma unroll
-      for (int j = 0; j < kAccFields; ++j) atomicAdd(&hvals[j][h], (double)v[j]);
-      return;
+      found = true;
+      break;
     }
     h = (h + 1) & (kHashSlots - 1);
   }
+  if (found) {
 #pragma unroll
-  for (int j = 0; j < kAccFields; ++j) add_global(A, p, j, v[j]);
+    for (int j = 0; j < kAccFields; ++j) atomicAdd(&hvals[j][h], (double)v[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kAccFields; ++j) add_global(A, p, j, v[j]);
+  }
 }
 
 __global__ void k_zero_f32(float* p, int64_t n) {
