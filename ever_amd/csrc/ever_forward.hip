@@ -304,11 +304,6 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
   S.tref = 0.f;
 }
 
-// the ray's proposal, if any (NaN or negative proposals are ignored)
-__device__ __forceinline__ void ray_bound(const FwdArgs& A, RayState& S) {
-  const float b = A.proposal[S.r];
-  S.bound = b >= 0.f ? b : INFINITY;
-}
 
 __device__ __forceinline__ bool ray_alive(const FwdArgs& A, const RayState& S) {
   return S.logT < (double)kLogCutoff && S.iter < A.max_iters;
@@ -334,6 +329,12 @@ __device__ __forceinline__ float scene_exit(const FwdArgs& A, const RayState& S)
     t = fminf(t, fmaxf(fmaf(lo, iv[a], -oiv[a]), fmaf(hi, iv[a], -oiv[a])));
   }
   return t;
+}
+
+// the ray's proposal, if any (NaN or negative proposals are ignored)
+__device__ __forceinline__ void ray_bound(const FwdArgs& A, RayState& S) {
+  const float b = A.proposal[S.r];
+  S.bound = b >= 0.f ? b : INFINITY;
 }
 
 // Leaf test of one primitive: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame, then the
@@ -527,7 +528,9 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
     uint64_t bk[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
-    const float cur_t = S.cur_t;
+    // the lower culling depth is the cursor, held at tmax once the cursor has passed it: exits beyond tmax whose
+    // entries lie before it may remain, and their boxes straddle tmax
+    const float cur_t = fminf(S.cur_t, A.tmax);
     // ---- traversal: gather the K nearest events after the cursor ----
     int sp = 0;
     int32_t node = root;
@@ -664,7 +667,9 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
 #pragma unroll
     for (int k = 0; k < K; ++k) ev[k][lane] = kNoEvent;
     uint64_t kmax = kNoEvent;
-    float t_lo = alive ? S.cur_t : INFINITY;  // a finished lane never enters a box
+    // a finished lane never enters a box; the cursor is held at tmax once past it (exits beyond tmax whose entries
+    // lie before it may remain, in boxes that straddle tmax)
+    float t_lo = alive ? fminf(S.cur_t, A.tmax) : INFINITY;
     // kept opaque: otherwise the select is folded into every box test below (two extra selects per child)
     asm volatile("" : "+v"(t_lo));
     int32_t stk0 = 0, stk1 = 0;                     // wave stack: entry i in lane i % 64 of stk(i / 64)

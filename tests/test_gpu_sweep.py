@@ -1,7 +1,8 @@
 """Randomised parity sweep of the HIP tracer against the FP64 oracle: seeded draws of scene kind and size,
 feature degree, camera (outside or inside the scene), tmin, tmax, max_iters, k-buffer size and tree walk, each
-checked forward (colour, opacity, distortion, per-ray event counts; 1e-4) and backward (every gradient tensor;
-1e-3 relative, or twice the error of the same algorithm evaluated in float32 where a case is worse conditioned).
+checked forward (colour 1e-4, opacity, distortion, per-ray event counts) and backward (every gradient tensor: 1e-3
+relative), each bar raised to twice the error of the same algorithm evaluated in float32 where a case is worse
+conditioned than that.
 Cases with a tmax that cuts through primitives and k-buffer sizes other than 16 check that results follow the
 reference's 16-event rounds whatever the buffer size."""
 import math
@@ -61,7 +62,7 @@ def _draw(seed):
                      f"tmax {tmax:.2f} max_iters {max_iters} K {kbuf} {traversal} wcts {wcts is not None}")
 
 
-@pytest.mark.parametrize("seed", range(32))
+@pytest.mark.parametrize("seed", range(96))
 def test_random_configuration_matches_oracle(seed):
     c = _draw(seed)
     print("case:", c["desc"], flush=True)
@@ -85,21 +86,22 @@ def test_random_configuration_matches_oracle(seed):
         fp = T.trace_forward(scene, og, dg, c["tmin"], c["tmax"], c["max_iters"], kbuf=c["kbuf"],
                              traversal=c["traversal"], proposal=prop.to(D))
         assert torch.equal(fp["color"], f["color"]) and torch.equal(fp["iters"], f["iters"])
-    col = f["color"].cpu().double().numpy()
-    err = np.abs(col[:, :4] - ref["color"][:, :4]).max()
-    derr = np.abs(col[:, 4] - ref["color"][:, 4]).max() / max(np.abs(ref["color"][:, 4]).max(), 1e-6)
-    iters = f["iters"].cpu().numpy()
-    mism = float((iters != ref["iters"]).mean())
-    print(f"colour max abs {err:.2e}, distortion rel {derr:.2e}, iters mismatch {mism:.4f}, "
-          f"events/ray {iters.mean():.1f}", flush=True)
-    assert err <= 1e-4 and derr <= 1e-4 and mism <= 0.005
-    w64 = None if c["wcts"] is None else c["wcts"].astype(np.float64)
-    rb = oracle.backward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["max_iters"],
-                         c["g"].double().numpy(), ref, wcts=w64)
     # the same algorithm evaluated in float32 on the CPU: its distance to the FP64 oracle is the conditioning of
     # the case at float32 precision, which no float32 implementation (the reference's included) goes below
     r32 = oracle.forward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["tmax"],
                          max_iters=c["max_iters"], use_bvh=True, dtype=np.float32)
+    col = f["color"].cpu().double().numpy()
+    err = np.abs(col[:, :4] - ref["color"][:, :4]).max()
+    f_err = np.abs(r32["color"][:, :4] - ref["color"][:, :4]).max()
+    derr = np.abs(col[:, 4] - ref["color"][:, 4]).max() / max(np.abs(ref["color"][:, 4]).max(), 1e-6)
+    iters = f["iters"].cpu().numpy()
+    mism = float((iters != ref["iters"]).mean())
+    print(f"colour max abs {err:.2e} (float32 restatement {f_err:.2e}), distortion rel {derr:.2e}, iters mismatch {mism:.4f}, "
+          f"events/ray {iters.mean():.1f}", flush=True)
+    assert err <= max(1e-4, 2.0 * f_err) and derr <= 1e-4 and mism <= 0.005, (err, f_err)
+    w64 = None if c["wcts"] is None else c["wcts"].astype(np.float64)
+    rb = oracle.backward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["max_iters"],
+                         c["g"].double().numpy(), ref, wcts=w64)
     b32 = oracle.backward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["max_iters"],
                           c["g"].double().numpy(), r32, wcts=w64, dtype=np.float32)
     for k in GRADS + (("dmeans2D",) if w64 is not None else ()):
