@@ -26,11 +26,12 @@ def _rel(a, b):
     return float((a - b).norm() / max(float(b.norm()), 1e-12))
 
 
-@pytest.mark.parametrize("tile", [0, 2])
-def test_batch_equals_per_camera_traces(tile):
-    print(f"case: 4 uneven cameras, tile {tile}", flush=True)
+@pytest.mark.parametrize("tile, tmax, kbuf", [(0, 100.0, 16), (2, 100.0, 16), (0, 4.2, 8), (1, 4.2, 32)])
+def test_batch_equals_per_camera_traces(tile, tmax, kbuf):
+    # a tmax of 4.2 from eyes at radius 4 cuts through the scene: exits beyond it follow the 16-event rounds
+    print(f"case: 4 uneven cameras, tile {tile}, tmax {tmax}, K {kbuf}", flush=True)
     t = _scene_tensors()
-    step = B.BatchStep(t, CAMS, kbuf=16, tile=tile)
+    step = B.BatchStep(t, CAMS, kbuf=kbuf, tile=tile, tmax=tmax)
     out = step.run()
     # per camera: its own single-view scene (SH at its eye) and its own trace
     # gradients are clamped per call (+-1e3, densities +-50): one call per camera clamps each camera's sum, the
@@ -44,10 +45,10 @@ def test_batch_equals_per_camera_traces(tile):
         sl = slice(start, start + nr)
         sc = T.Scene(*t, color_mode=1, sh_origin=eye, sh_degree=3, activation=1).build()
         o, d = step.rayo[sl].contiguous(), step.rayd[sl].contiguous()
-        f = T.trace_forward(sc, o, d, 0.0, 100.0, 500, traversal="ray")
+        f = T.trace_forward(sc, o, d, 0.0, tmax, 500, traversal="ray")
         assert torch.equal(f["color"], out["color"][sl]), v
         assert torch.equal(f["iters"], out["iters"][sl]), v
-        b = T.trace_backward(sc, o, d, 0.0, 100.0, 500, f, step.grad[sl].contiguous())
+        b = T.trace_backward(sc, o, d, 0.0, tmax, 500, f, step.grad[sl].contiguous())
         for k in sums:
             sums[k] += b[k]
             clamped[k] |= b[k].abs() >= 0.999 * limit[k]
