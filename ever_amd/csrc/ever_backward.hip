@@ -451,27 +451,42 @@ __global__ __launch_bounds__(256) void k_sh_feature_grads(const float* __restric
                                                           const float* __restrict__ means, float ox, float oy,
                                                           float oz, int64_t n, int d, int deg,
                                                           float* __restrict__ dfeat) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = 3 * (int64_t)d;
-  if (e >= n * per) return;
-  const int64_t p = e / per;
-  const int j = (int)(e - p * per);
-  const int k = j / 3, ch = j - 3 * k;
+  // one lane per primitive: the basis once, then its d x 3 gradients (coefficients beyond the degree are zero)
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
   const int nc = (deg + 1) * (deg + 1);
-  float v = 0.f;
-  if (k < nc) {
-    const float g = grgb[4 * p + ch];
-    if (g != 0.f) {
-      const f3 dir = sh_dir(ld3(means + 3 * p), mk3(ox, oy, oz));
-      float bas[16];
-      sh_basis(deg, dir.x, dir.y, dir.z, bas);
-      float bk = bas[0];
+  const float4 g = reinterpret_cast<const float4*>(grgb)[p];
+  float bas[16];
+  if (g.x != 0.f || g.y != 0.f || g.z != 0.f) {
+    const f3 dir = sh_dir(ld3(means + 3 * p), mk3(ox, oy, oz));
+    sh_basis(deg, dir.x, dir.y, dir.z, bas);
+  } else {
 #pragma unroll
-      for (int t = 1; t < 16; ++t) bk = (t == k) ? bas[t] : bk;
-      v = bk * g;
+    for (int t = 0; t < 16; ++t) bas[t] = 0.f;
+  }
+  float* out = dfeat + 3 * (int64_t)d * p;
+  if (d == 16 && (reinterpret_cast<uintptr_t>(dfeat) & 15u) == 0) {  // 192 B per primitive: twelve vector stores
+    float v[48];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const float b = t < nc ? bas[t] : 0.f;
+      v[3 * t] = b * g.x;
+      v[3 * t + 1] = b * g.y;
+      v[3 * t + 2] = b * g.z;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      reinterpret_cast<float4*>(out)[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  } else {
+    for (int t = 0; t < d; ++t) {
+      float b = 0.f;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) b = (u == t && t < nc) ? bas[u] : b;
+      out[3 * t] = b * g.x;
+      out[3 * t + 1] = b * g.y;
+      out[3 * t + 2] = b * g.z;
     }
   }
-  dfeat[e] = v;
 }
 
 // Several views: dL/df[p,k,c] = sum_v Y_k(dir_{p,v}) * g_v[p,c], one lane per primitive.
@@ -669,7 +684,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
         hipLaunchKernelGGL(k_sh_feature_grads_views, dim3(grid_for(n, 256)), dim3(256), 0, st, B.view_color_grads,
                            P.means, P.view_origins, P.num_views, n, P.feature_size, P.sh_degree, B.dfeatures);
       else
-        hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n * 3 * (int64_t)P.feature_size, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n, 256)), dim3(256), 0,
                            st, E.grgb, P.means, P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size,
                            P.sh_degree, B.dfeatures);
     }
