@@ -193,3 +193,16 @@ def test_last_dirac_is_the_last_integrated_control_point(traversal):
         want[r] = [h, h * col[ev >> 1, 0], h * col[ev >> 1, 1], h * col[ev >> 1, 2]]
     assert (it == 0).any() and (it > 0).any()
     np.testing.assert_allclose(out["last_dirac"].cpu().numpy(), want, rtol=1e-6, atol=1e-6)
+
+
+def test_three_million_primitives_match_oracle():
+    # a large scene: 12M leaf slots in the sort and the tree (int32 indices throughout), 53 events per ray
+    print("case: 3M small primitives, 48x48", flush=True)
+    sc = scenes.uniform_scene(3_000_000, 0, seed=41)
+    sc["scales"] = (sc["scales"] * 0.2).contiguous()
+    o, d = scenes.pinhole_rays(48, 48, (0.0, 0.0, -3.0))
+    ref = oracle_forward(sc, o, d, use_bvh=True)
+    for traversal in ("ray", "wave"):
+        _, out = gpu_forward(sc, o, d, traversal=traversal)
+        assert (_np(out["iters"]) != ref["iters"]).mean() <= 1e-3
+        np.testing.assert_allclose(_np(out["color"])[:, :4], ref["color"][:, :4], atol=1e-4, rtol=1e-4)
