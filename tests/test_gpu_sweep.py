@@ -93,12 +93,14 @@ def test_random_configuration_matches_oracle(seed):
     col = f["color"].cpu().double().numpy()
     err = np.abs(col[:, :4] - ref["color"][:, :4]).max()
     f_err = np.abs(r32["color"][:, :4] - ref["color"][:, :4]).max()
-    derr = np.abs(col[:, 4] - ref["color"][:, 4]).max() / max(np.abs(ref["color"][:, 4]).max(), 1e-6)
+    dden = max(np.abs(ref["color"][:, 4]).max(), 1e-6)
+    derr = np.abs(col[:, 4] - ref["color"][:, 4]).max() / dden
+    f_derr = np.abs(r32["color"][:, 4] - ref["color"][:, 4]).max() / dden
     iters = f["iters"].cpu().numpy()
     mism = float((iters != ref["iters"]).mean())
-    print(f"colour max abs {err:.2e} (float32 restatement {f_err:.2e}), distortion rel {derr:.2e}, iters mismatch {mism:.4f}, "
-          f"events/ray {iters.mean():.1f}", flush=True)
-    assert err <= max(1e-4, 2.0 * f_err) and derr <= 1e-4 and mism <= 0.005, (err, f_err)
+    print(f"colour max abs {err:.2e} (float32 restatement {f_err:.2e}), distortion rel {derr:.2e} ({f_derr:.2e}), "
+          f"iters mismatch {mism:.4f}, events/ray {iters.mean():.1f}", flush=True)
+    assert err <= max(1e-4, 2.0 * f_err) and derr <= max(1e-4, 2.0 * f_derr) and mism <= 0.005, (err, f_err, derr)
     w64 = None if c["wcts"] is None else c["wcts"].astype(np.float64)
     rb = oracle.backward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["max_iters"],
                          c["g"].double().numpy(), ref, wcts=w64)
