@@ -395,3 +395,129 @@ def test_buffer_rounds_matter_at_a_cutting_tmax():
     a = oracle.forward(*sc, o, d, c["tmin"], 100.0, max_iters=c["max_iters"], use_bvh=True, kbuf=16)
     b = oracle.forward(*sc, o, d, c["tmin"], 100.0, max_iters=c["max_iters"], use_bvh=True, kbuf=19)
     assert np.array_equal(a["iters"], b["iters"]) and np.abs(a["color"] - b["color"]).max() < 1e-12
+
+
+def _real_sh_scipy(dirs):
+    """The 16 real SH basis functions of degrees 0-3 at unit directions, from scipy's complex Y_l^m (which carries
+    the Condon-Shortley phase): index l*l + l + m holds Y_l^0 for m = 0, sqrt(2) Re Y_l^m for m > 0 and
+    sqrt(2) Im Y_l^|m| for m < 0. Keeping the phase is what gives the PlenOctree signs of sh.slang:48-88 (for
+    example -C1*y, C1*z, -C1*x at degree 1)."""
+    from scipy.special import sph_harm_y
+    th = np.arccos(np.clip(dirs[:, 2], -1.0, 1.0))
+    ph = np.arctan2(dirs[:, 1], dirs[:, 0])
+    S = np.zeros((dirs.shape[0], 16))
+    for l in range(4):
+        for m in range(-l, l + 1):
+            Y = sph_harm_y(l, abs(m), th, ph)
+            S[:, l * l + l + m] = Y.real if m == 0 else np.sqrt(2) * (Y.imag if m < 0 else Y.real)
+    return S
+
+
+def _oracle_sh_basis(dirs):
+    # basis function k = colour of a one-hot feature on coefficient k, minus the 0.5 offset (sh.slang:48-51)
+    B = np.zeros((dirs.shape[0], 16))
+    for k in range(16):
+        f = np.zeros((dirs.shape[0], 16, 3))
+        f[:, k, 0] = 1.0
+        B[:, k] = oracle.eval_sh(dirs, f, 3, np.zeros(3))[:, 0] - 0.5
+    return B
+
+
+def test_sh_basis_matches_scipy_real_harmonics():
+    # all 16 basis functions of sh.slang:48-88 (constants third_party/sh_util.py:26-54) against scipy at generic
+    # directions; a flipped sign or two coefficients swapped within a degree must fail this comparison
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(32, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    B, S = _oracle_sh_basis(v), _real_sh_scipy(v)
+    np.testing.assert_allclose(B, S, rtol=0, atol=1e-13)
+    # the comparison separates every such mistake: each single sign flip and each within-degree swap moves some
+    # basis value by far more than the tolerance
+    for k in range(1, 16):
+        assert np.abs(-S[:, k] - S[:, k]).max() > 1e-2
+    for l in range(1, 4):
+        for i in range(l * l, (l + 1) ** 2):
+            for j in range(i + 1, (l + 1) ** 2):
+                assert np.abs(S[:, i] - S[:, j]).max() > 1e-2
+
+
+def _means2d_scene(seed=3, n=8, r=12):
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(-0.4, 0.4, (n, 3))
+    scales = rng.uniform(0.15, 0.4, (n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    dens = rng.uniform(0.5, 3, n)
+    feats = rng.normal(0, 0.5, (n, 1, 3))
+    eye = np.array([0.3, -0.2, -3.0])
+    rayo = np.repeat(eye[None], r, 0) + rng.normal(0, 0.01, (r, 3))
+    rayd = np.c_[rng.uniform(-0.15, 0.15, (r, 2)), np.ones(r)]
+    g = rng.normal(size=(r, 5))
+    return [means, scales, q, dens, feats, rayo, rayd], g
+
+
+def _wct(rng):
+    # world-to-camera in the row-vector convention of backwards_kernel.slang:244-250 (p_view = [x, y, z, 1] @ W)
+    a = rng.normal(size=(3, 3))
+    Rm, _ = np.linalg.qr(a)
+    W = np.eye(4)
+    W[:3, :3] = Rm
+    W[3, :3] = rng.normal(0, 0.3, 3) + np.array([0.1, -0.2, 3.5])  # translation keeps the scene in front
+    return W
+
+
+def _inv_project_vjp(mean, W, upstream, transpose=False):
+    """bwd_diff(inv_project) w.r.t. xy (backwards_kernel.slang:213-220, 244-257) by float64 torch autograd: xy and
+    dist come from project(mean, W); inv_project(xy, dist, inv(W)) = ([xy * dist, dist, 1] @ inv(W))[:3]."""
+    import torch
+    Wt = torch.tensor(W, dtype=torch.float64)
+    m = torch.tensor(mean, dtype=torch.float64)
+    pv = torch.cat([m, torch.ones(1, dtype=torch.float64)]) @ Wt
+    xy = (pv[:2] / pv[2]).detach().requires_grad_(True)
+    dist = pv[2].detach()
+    inv = torch.linalg.inv(Wt)
+    if transpose:
+        inv = inv.T
+    p_hom = torch.cat([xy * dist, dist.reshape(1), torch.ones(1, dtype=torch.float64)])
+    out = (p_hom @ inv)[:3]
+    out.backward(torch.tensor(upstream, dtype=torch.float64))
+    return xy.grad.numpy()
+
+
+def test_means2d_statistic_one_matrix_matches_autograd():
+    params, g = _means2d_scene()
+    W = _wct(np.random.default_rng(4))
+    f = oracle.forward(*params, 0.0, 100.0)
+    b = oracle.backward(*params, 0.0, 500, g, f, wcts=W[None], clamp=False)
+    assert np.abs(b["dmeans"]).max() > 1e-3
+    # one matrix: every event of primitive p has the same Jacobian, so the statistic is its transpose applied to
+    # the summed dL/dmean
+    exp = np.stack([_inv_project_vjp(params[0][p], W, b["dmeans"][p]) for p in range(params[0].shape[0])])
+    np.testing.assert_allclose(b["dmeans2D"], exp, rtol=1e-10, atol=1e-12)
+    wrong = np.stack([_inv_project_vjp(params[0][p], W, b["dmeans"][p], transpose=True)
+                      for p in range(params[0].shape[0])])
+    assert np.abs(wrong - exp).max() > 1e-2 * np.abs(exp).max()
+
+
+def test_means2d_statistic_per_ray_matrices_matches_autograd():
+    params, g = _means2d_scene(seed=5)
+    rng = np.random.default_rng(6)
+    nr = params[5].shape[0]
+    Ws = np.stack([_wct(rng) for _ in range(nr)])
+    f = oracle.forward(*params, 0.0, 100.0)
+    b = oracle.backward(*params, 0.0, 500, g, f, wcts=Ws, clamp=False)
+    n = params[0].shape[0]
+    exp = np.zeros((n, 2))
+    wrong = np.zeros((n, 2))
+    for r in range(nr):
+        # the dL/dmean that ray r alone contributes, and its Jacobian through ray r's matrix
+        gr = np.zeros_like(g)
+        gr[r] = g[r]
+        br = oracle.backward(*params, 0.0, 500, gr, f, clamp=False)
+        for p in range(n):
+            if np.any(br["dmeans"][p] != 0):
+                exp[p] += _inv_project_vjp(params[0][p], Ws[r], br["dmeans"][p])
+                wrong[p] += _inv_project_vjp(params[0][p], Ws[r], br["dmeans"][p], transpose=True)
+    assert np.abs(exp).max() > 1e-3
+    np.testing.assert_allclose(b["dmeans2D"], exp, rtol=1e-9, atol=1e-12)
+    assert np.abs(wrong - exp).max() > 1e-2 * np.abs(exp).max()
