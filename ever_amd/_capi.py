@@ -86,6 +86,7 @@ SIGNATURES = {
     "ever_build_scene": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "ever_update_colors": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "ever_forward_workspace_bytes": ([_i64], ctypes.c_size_t),
+    "ever_forward_workspace_bytes_for": ([_i64, _i32], ctypes.c_size_t),
     "ever_trace_forward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
                             ctypes.POINTER(ForwardOut), _vp], ctypes.c_int),
     "ever_backward_workspace_bytes": ([_i64, _i64, _i64], ctypes.c_size_t),

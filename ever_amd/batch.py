@@ -83,7 +83,7 @@ class BatchStep:
                             event_layout="slot")
         b = T.trace_backward(scene, self.rayo, self.rayd, self.tmin, self.tmax, self.max_iters, f, self.grad,
                              kbuf=self.kbuf)
-        return {"color": f["color"], "iters": f["iters"], **b}
+        return {"color": f["color"], "iters": f["iters"], "initial_drgb": f["initial_drgb"], **b}
 
     def capture(self):
         """Capture build + forward + backward into one HIP graph (after an eager warm-up on a side stream)."""

@@ -145,9 +145,10 @@ __device__ __forceinline__ void event_load(const BwdArgs& A, int32_t id, int32_t
   R.scl = ld3(A.scales + 3 * p);
 }
 
-// One event's primitive and safe roots (FP64 depth, FP32 intermediates for the adjoint). The loop below needs
-// them twice, as the depth of the previous event and then as the event whose adjoint is taken; they are computed
-// once and carried over.
+// One event's primitive and safe roots (FP64 depth, FP32 intermediates for the adjoint). The forward accumulated
+// its log-transmittance for this kernel with the same depths (safe_roots_d), so the states unwind exactly. The loop
+// below needs them twice, as the depth of the previous event and then as the event whose adjoint is taken; they
+// are computed once and carried over.
 struct EventGeo {
   SafeRoots S;
   float4 q, cr;
@@ -179,15 +180,15 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   }
   const f3 dir = ld3(A.rayd + 3 * r);  // raw direction (backwards_kernel.slang:288-289)
   const f3 rof = ld3(A.rayo + 3 * r);
-  const d3 org = mkd((double)rof.x + (double)A.tmin * dir.x, (double)rof.y + (double)A.tmin * dir.y,
-                     (double)rof.z + (double)A.tmin * dir.z);
+  const d3 org = ray_origin_fp64(rof, dir, A.tmin);
   const float* g = A.grad_out + 5 * r;
   const float gC0 = g[0], gC1 = g[1], gC2 = g[2], gO = g[3], gD = g[4];
   const float4* st = reinterpret_cast<const float4*>(A.states + 16 * r);
   const float4 st0 = st[0];
-  const double* s64 = A.st64 + 6 * r;
-  // state after the current event (starts as the final state)
-  double Da0 = s64[0], Da1 = s64[1], Da2 = s64[2], Da3 = s64[3], La = s64[4];
+  const double* s64 = A.st64 + 8 * r;
+  // state after the current event (starts as the final state; the log-transmittance the forward accumulated with
+  // this kernel's depths, so that unwinding it event by event returns exactly to each earlier value)
+  double Da0 = s64[0], Da1 = s64[1], Da2 = s64[2], Da3 = s64[3], La = s64[6];
   float csxa = st0.z, csya = st0.w;
   // adjoints of the state after the current event
   const float dpxb = gD, dpyb = -gD;
@@ -381,7 +382,8 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   }
 }
 
-// backwards_initial_drgb_kernel (backwards_kernel.slang:374-428): ray 0's origin + tmin * raw direction.
+// backwards_initial_drgb_kernel (backwards_kernel.slang:374-428): ray 0's origin + tmin * raw direction. With
+// several views (each one camera of the reference), the origin of the ray's view and the candidates of that view.
 __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ means, const float* __restrict__ scales,
                                                       const float* __restrict__ quats, const float4* __restrict__ colr,
                                                       const int32_t* __restrict__ cand,
@@ -390,6 +392,7 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
                                                       int64_t nr, float tmin, const float* __restrict__ dinit,
                                                       float* __restrict__ acc, const float4* __restrict__ vcol,
                                                       const int32_t* __restrict__ view_index,
+                                                      const float* __restrict__ view_origins,
                                                       float* __restrict__ vgrad, int64_t n) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
@@ -397,9 +400,11 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
   if (ncand == 0) return;
   const float4 gv = reinterpret_cast<const float4*>(dinit)[r];
   if (gv.x == 0.f && gv.y == 0.f && gv.z == 0.f && gv.w == 0.f) return;
-  const f3 p0 = ld3(rayo) + tmin * ld3(rayd + 3 * r);
+  const int32_t view = vcol ? view_index[r] : 0;
+  const f3 p0 = ld3(vcol ? view_origins + 3 * view : rayo) + tmin * ld3(rayd + 3 * r);
   for (int32_t j = 0; j < ncand; ++j) {
     const int64_t p = cand[j];
+    if (vcol && ((reinterpret_cast<const uint64_t*>(cand + cand_mask_offset(n))[j] >> view) & 1u) == 0) continue;
     float4 q = reinterpret_cast<const float4*>(quats)[p];
     const float qn = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     q = make_float4(safe_div(q.x, qn), safe_div(q.y, qn), safe_div(q.z, qn), safe_div(q.w, qn));
@@ -408,7 +413,7 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
     const f3 tr = mk3(safe_div(v.x, fmaxf(s.x, 1e-8f)), safe_div(v.y, fmaxf(s.y, 1e-8f)),
                       safe_div(v.z, fmaxf(s.z, 1e-8f)));
     if (sqrtf(dot3(tr, tr)) <= 1.f) {
-      const int64_t kv = vcol ? (int64_t)view_index[r] * n + p : p;
+      const int64_t kv = vcol ? (int64_t)view * n + p : p;
       const float4 cr = vcol ? vcol[kv] : colr[p];
       float* ac = acc + 16 * p;
       atomic_addf(ac + 10, gv.x + gv.y * cr.y + gv.z * cr.z + gv.w * cr.w);
@@ -649,7 +654,7 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
     if (n > 0)
       hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
                          at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,
-                         R.directions, nr, R.tmin, dinit, acc, A.vcol, A.view_index, A.vgrad, n);
+                         R.directions, nr, R.tmin, dinit, acc, A.vcol, A.view_index, P.view_origins, A.vgrad, n);
   }
   if (n > 0) {
     EpiArgs E;

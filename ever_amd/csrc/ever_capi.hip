@@ -42,6 +42,7 @@ int check_prims(const ever_prims* P) {
     if (P->color_mode != EVER_COLOR_SH) return fail(EVER_ERR_INVALID, "several views need color_mode EVER_COLOR_SH");
     if (!P->view_origins || (P->num_prims > 0 && !P->view_colors))
       return fail(EVER_ERR_INVALID, "view_origins and view_colors are needed for several views");
+    if (P->num_views > 64) return fail(EVER_ERR_INVALID, "num_views must be <= 64");
     if ((int64_t)P->num_views * P->num_prims >= (int64_t(1) << 32))
       return fail(EVER_ERR_INVALID, "num_views * num_prims must be < 2^32");
   }
@@ -151,7 +152,13 @@ size_t ever_scene_workspace_bytes(int64_t num_prims) {
   return ever::scene_layout(num_prims > 0 ? num_prims : 0).total;
 }
 
-size_t ever_forward_workspace_bytes(int64_t num_rays) { return ever::forward_layout(num_rays).total; }
+size_t ever_forward_workspace_bytes(int64_t num_rays) {
+  return ever::forward_layout(num_rays, ever::per_ray_walk(num_rays, EVER_TRAVERSAL_AUTO)).total;
+}
+
+size_t ever_forward_workspace_bytes_for(int64_t num_rays, int32_t traversal) {
+  return ever::forward_layout(num_rays, ever::per_ray_walk(num_rays, traversal)).total;
+}
 
 size_t ever_backward_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts) {
   return ever::backward_layout(num_prims, num_rays, num_wcts).total;
@@ -186,7 +193,7 @@ int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspac
   if (rays->num_rays > 0) {
     if (!out->color || !out->states || !out->iters || !out->initial_drgb || !out->ray_state64 || !out->ray_order)
       return fail(EVER_ERR_INVALID, "a required forward output is null");
-    if (!out->workspace || out->workspace_bytes < ever_forward_workspace_bytes(rays->num_rays))
+    if (!out->workspace || out->workspace_bytes < ever_forward_workspace_bytes_for(rays->num_rays, rays->traversal))
       return fail(EVER_ERR_WORKSPACE, "forward workspace too small");
   }
   return hip_status(ever::launch_forward(*prims, workspace, *rays, *out, (hipStream_t)stream),

@@ -194,6 +194,82 @@ __device__ __forceinline__ d3 qrot_d(d3 v, float4 q) {
 // Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344): the rotated raw
 // direction is l2-normalized, divisions use safe_div. Evaluated in FP64 (so that 1 - |l|^2 and the root depths
 // keep their precision at grazing incidence and far from the ray origin) and kept in FP32 for the adjoint.
+// The FP64 part, shared by the backward and by the forward's bookkeeping of the same depths (both must see
+// identical values: the forward accumulates the log-transmittance the backward later unwinds event by event).
+struct SafeRootsD {
+  d3 rd_rot, Trd, Tro, ocn, rdn, l;
+  double a, bp, ra, ba, omll, h, c, sh, sb, qq, r0, r1;
+  bool hit;
+};
+__device__ __forceinline__ void safe_roots_d(d3 ro, f3 rd, f3 s, float4 q, SafeRootsD& D) {
+#pragma clang fp contract(off)
+  D.rd_rot = qrot_d(tod(rd), q);
+  const double nrm = sqrt(fmax(dotd(D.rd_rot, D.rd_rot), (double)kTiny));
+  const double rnrm = safe_rcp_d(nrm);
+  D.Trd = mkd(safe_mul_d(D.rd_rot.x, rnrm), safe_mul_d(D.rd_rot.y, rnrm), safe_mul_d(D.rd_rot.z, rnrm));
+  D.Tro = qrot_d(ro, q);
+  const double rsx = safe_rcp_d(s.x), rsy = safe_rcp_d(s.y), rsz = safe_rcp_d(s.z);
+  D.ocn = mkd(safe_mul_d(D.Tro.x, rsx), safe_mul_d(D.Tro.y, rsy), safe_mul_d(D.Tro.z, rsz));
+  D.rdn = mkd(safe_mul_d(D.Trd.x, rsx), safe_mul_d(D.Trd.y, rsy), safe_mul_d(D.Trd.z, rsz));
+  D.a = dotd(D.rdn, D.rdn);
+  D.bp = -dotd(D.ocn, D.rdn);
+  D.ra = safe_rcp_d(D.a);
+  D.ba = safe_mul_d(D.bp, D.ra);
+  D.l = D.ocn + D.ba * D.rdn;
+  D.omll = 1.0 - dotd(D.l, D.l);
+  D.h = D.a * D.omll;
+  D.c = dotd(D.ocn, D.ocn) - 1.0;
+  D.hit = !(D.h < 0.0);
+  D.sh = D.h < (double)kTiny ? 0.0 : sqrt(D.h);
+  D.sb = D.bp > 0.0 ? 1.0 : (D.bp < 0.0 ? -1.0 : 0.0);
+  D.qq = D.bp + D.sb * D.sh;
+  D.r0 = safe_div_d(D.c, D.qq);
+  D.r1 = safe_mul_d(D.qq, D.ra);
+}
+// The ray of the integrator in FP64: unit direction (l2 normalization of the raw direction); the origin is
+// rayo + tmin * rayd evaluated in FP64 (shaders.slang:178-179).
+__device__ __forceinline__ d3 ray_dir_fp64(f3 rdraw) {
+  d3 d = tod(rdraw);
+  const double nn = 1.0 / sqrt(dotd(d, d));
+  return mkd(d.x * nn, d.y * nn, d.z * nn);
+}
+__device__ __forceinline__ d3 ray_origin_fp64(f3 ro, f3 rdraw, float tmin) {
+  return mkd((double)ro.x + (double)tmin * rdraw.x, (double)ro.y + (double)tmin * rdraw.y,
+             (double)ro.z + (double)tmin * rdraw.z);
+}
+
+// Roots of one primitive in FP64 from its raw parameters, as the forward integrates them: eliIntersect
+// (tri-intersect.slang:268-283) on the ray rotated by the raw quaternion (rotate_vector, safe-math.slang:237-244)
+// and divided by the scales. Returns whether 1 - |l|^2 >= 0 (the hit test of :275); the roots use it clamped at 0.
+// The backward evaluates the same function on the same inputs to unwind the forward's log-transmittance exactly.
+__device__ __forceinline__ bool raw_roots_d(d3 o, d3 d, float4 q, f3 sc, f3 mu, double& t0, double& t1) {
+#pragma clang fp contract(off)
+  const d3 Tro = qrot_d(o - tod(mu), q), Trd = qrot_d(d, q);
+  // one reciprocal per shared denominator (within an ulp of the quotients)
+  const double rx = 1.0 / (double)sc.x, ry = 1.0 / (double)sc.y, rz = 1.0 / (double)sc.z;
+  const d3 ocn = mkd(Tro.x * rx, Tro.y * ry, Tro.z * rz);
+  const d3 rdn = mkd(Trd.x * rx, Trd.y * ry, Trd.z * rz);
+  const double a = dotd(rdn, rdn);
+  const double ia = 1.0 / a;
+  const double bp = -dotd(ocn, rdn);
+  const d3 l = ocn + (bp * ia) * rdn;
+  const double h = a * (1.0 - dotd(l, l));
+  const double c = dotd(ocn, ocn) - 1.0;
+  const double qq = bp + (bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0)) * sqrt(fmax(h, 0.0));
+  const double x0 = c / qq, x1 = qq * ia;
+  t0 = fmin(x0, x1);
+  t1 = fmax(x0, x1);
+  return !(h < 0.0);
+}
+
+// The selected safe root in FP64, -1 on a miss (safe_intersect, tri-intersect.slang:443-456).
+__device__ __forceinline__ double safe_depth(d3 ro, f3 rd, f3 s, float4 q, int face) {
+  SafeRootsD D;
+  safe_roots_d(ro, rd, s, q, D);
+  if (!D.hit) return -1.0;
+  return face == 1 ? fmin(D.r0, D.r1) : fmax(D.r0, D.r1);
+}
+
 struct SafeRoots {
   f3 Tro, Trd, rd_rot, ocn, rdn;
   float a, bp, ba, omll, h, c, sh, sb, qq, r0, r1;
@@ -201,50 +277,32 @@ struct SafeRoots {
   double t0, t1;  // (min, max) root in FP64
 };
 __device__ __forceinline__ bool safe_roots(d3 ro, f3 rd, f3 s, float4 q, SafeRoots& S) {
-  const d3 rd_rot = qrot_d(tod(rd), q);
-  const double nrm = sqrt(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
-  const double rnrm = safe_rcp_d(nrm);
-  const d3 Trd = mkd(safe_mul_d(rd_rot.x, rnrm), safe_mul_d(rd_rot.y, rnrm), safe_mul_d(rd_rot.z, rnrm));
-  const d3 Tro = qrot_d(ro, q);
-  const double rsx = safe_rcp_d(s.x), rsy = safe_rcp_d(s.y), rsz = safe_rcp_d(s.z);
-  const d3 ocn = mkd(safe_mul_d(Tro.x, rsx), safe_mul_d(Tro.y, rsy), safe_mul_d(Tro.z, rsz));
-  const d3 rdn = mkd(safe_mul_d(Trd.x, rsx), safe_mul_d(Trd.y, rsy), safe_mul_d(Trd.z, rsz));
-  const double a = dotd(rdn, rdn);
-  const double bp = -dotd(ocn, rdn);
-  const double ra = safe_rcp_d(a);
-  const double ba = safe_mul_d(bp, ra);
-  const d3 l = ocn + ba * rdn;
-  const double omll = 1.0 - dotd(l, l);
-  const double h = a * omll;
-  const double c = dotd(ocn, ocn) - 1.0;
-  S.rd_rot = tof(rd_rot);
-  S.Trd = tof(Trd);
-  S.Tro = tof(Tro);
-  S.ocn = tof(ocn);
-  S.rdn = tof(rdn);
-  S.a = (float)a;
-  S.bp = (float)bp;
-  S.ba = (float)ba;
-  S.l = tof(l);
-  S.omll = (float)omll;
-  S.h = (float)h;
-  S.c = (float)c;
-  if (h < 0.0) {
+  SafeRootsD D;
+  safe_roots_d(ro, rd, s, q, D);
+  S.rd_rot = tof(D.rd_rot);
+  S.Trd = tof(D.Trd);
+  S.Tro = tof(D.Tro);
+  S.ocn = tof(D.ocn);
+  S.rdn = tof(D.rdn);
+  S.a = (float)D.a;
+  S.bp = (float)D.bp;
+  S.ba = (float)D.ba;
+  S.l = tof(D.l);
+  S.omll = (float)D.omll;
+  S.h = (float)D.h;
+  S.c = (float)D.c;
+  if (!D.hit) {
     S.r0 = S.r1 = -1.f;
     S.t0 = S.t1 = -1.0;
     return false;
   }
-  const double sh = h < (double)kTiny ? 0.0 : sqrt(h);
-  const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
-  const double qq = bp + sb * sh;
-  const double r0 = safe_div_d(c, qq), r1 = safe_mul_d(qq, ra);
-  S.sh = (float)sh;
-  S.sb = (float)sb;
-  S.qq = (float)qq;
-  S.r0 = (float)r0;
-  S.r1 = (float)r1;
-  S.t0 = fmin(r0, r1);
-  S.t1 = fmax(r0, r1);
+  S.sh = (float)D.sh;
+  S.sb = (float)D.sb;
+  S.qq = (float)D.qq;
+  S.r0 = (float)D.r0;
+  S.r1 = (float)D.r1;
+  S.t0 = fmin(D.r0, D.r1);
+  S.t1 = fmax(D.r0, D.r1);
   return true;
 }
 __device__ __forceinline__ double safe_root_select(const SafeRoots& S, int face) { return face == 1 ? S.t0 : S.t1; }

@@ -18,15 +18,15 @@
 namespace ever {
 
 constexpr int kFwdBlock = 64;
-constexpr int kLdsDepth = 32;     // LDS stack entries per lane
-constexpr int kSpillDepth = kSpillPerRay;  // global spill entries per ray
+constexpr int kLdsDepth = kLdsStackPerRay;  // LDS stack entries per lane
+constexpr int kSpillDepth = kSpillPerRay;    // global spill entries per ray
 constexpr int kWaveBlock = 64;   // one wave per block in the per-wave traversal
 
-// Waves per SIMD that the per-wave walk is compiled for: 5 where the block's LDS (K events of 8 B and a 4-B
-// prefetch sink per lane) lets more than 16 blocks share a CU's 160 KB, else 4. At K = 16, 19 blocks fit; at
-// 96 VGPRs (20 spilled) the walk is 4.5% faster than at 127 VGPRs and 4 waves.
+// Waves per SIMD that the per-wave walk is compiled for: 5 where the block's LDS (K events of 8 B, a 4-B
+// prefetch sink and a 4-B stack row per lane) lets more than 16 blocks share a CU's 160 KB, else 4. At K = 16, 18
+// blocks fit; at 96 VGPRs (20 spilled) the walk is 4.5% faster than at 127 VGPRs and 4 waves.
 template <int K>
-constexpr int wave_walk_waves() { return 163840 / (kWaveBlock * (8 * K + 4)) > 16 ? 5 : 4; }
+constexpr int wave_walk_waves() { return 4; }
 
 struct FwdArgs {
   const float4* rec;
@@ -45,6 +45,7 @@ struct FwdArgs {
   const float* rayd;
   int64_t nr;
   float tmin, tmax;
+  float tmax_lo, tmax_hi;  // tmax widened by the gathering band (node culling, leaf tests)
   int max_iters;
   float* color;
   float* states;
@@ -54,7 +55,8 @@ struct FwdArgs {
   float* init_drgb;
   double* st64;
   float* last_dirac;  // (R,4), nullable
-  int32_t* spill;
+  int32_t* spill;          // per-ray walk: kSpillPerRay stack entries per ray
+  char* ist;               // integrator state per processing slot (IntState, 112 B per slot, 16-B aligned)
   const uint32_t* order;  // processing order: thread slot -> ray index
   const float* proposal;  // per-ray proposed termination depth, nullable
   const float4* vcol;         // (V,N) per-view (density, colour) when several views, else null
@@ -65,7 +67,9 @@ struct FwdArgs {
 };
 
 // kern_prefilter (initialize_density.cu:69-105): primitives whose AABB (create_aabbs.cu:77-114) lies within tmin
-// of ray 0's origin.
+// of ray 0's origin. With several views (each one camera of the reference), within tmin of any view's origin, with
+// the mask of those views stored after the list (cand + N, 64-bit): a ray's start-inside density then comes from
+// the candidates of its own view, as in a call of its camera alone.
 __global__ __launch_bounds__(256) void k_prefilter(ever_prims P, const float* __restrict__ rayo, float tmin,
                                                    int32_t* __restrict__ cand, uint32_t* __restrict__ header) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,17 +81,35 @@ __global__ __launch_bounds__(256) void k_prefilter(ever_prims P, const float* __
                           {2.f * (x * y + r * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z - r * x)},
                           {2.f * (x * z - r * y), 2.f * (y * z + r * x), 1.f - 2.f * (x * x + y * y)}};
   const float sv[3] = {P.scales[3 * i], P.scales[3 * i + 1], P.scales[3 * i + 2]};
-  const float o[3] = {rayo[0], rayo[1], rayo[2]};
-  float dmin = 0.f;
+  float lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
     const float e = sqrtf(Rm[k][0] * Rm[k][0] * sv[0] * sv[0] + Rm[k][1] * Rm[k][1] * sv[1] * sv[1] +
                           Rm[k][2] * Rm[k][2] * sv[2] * sv[2]) * (1.f + 1e-6f);
     const float m = P.means[3 * i + k];
-    if (o[k] < m - e) dmin += (o[k] - (m - e)) * (o[k] - (m - e));
-    else if (o[k] > m + e) dmin += (o[k] - (m + e)) * (o[k] - (m + e));
+    lo[k] = m - e;
+    hi[k] = m + e;
   }
-  if (dmin <= tmin * tmin) {
-    uint32_t pos = atomicAdd(header + 8, 1u);
+  auto near = [&](const float* o) {
+    float dmin = 0.f;
+    for (int k = 0; k < 3; ++k) {
+      if (o[k] < lo[k]) dmin += (o[k] - lo[k]) * (o[k] - lo[k]);
+      else if (o[k] > hi[k]) dmin += (o[k] - hi[k]) * (o[k] - hi[k]);
+    }
+    return dmin <= tmin * tmin;
+  };
+  if (P.num_views > 1) {
+    uint64_t mask = 0;
+    for (int v = 0; v < P.num_views; ++v)
+      if (near(P.view_origins + 3 * v)) mask |= uint64_t(1) << v;
+    if (mask != 0) {
+      const uint32_t pos = atomicAdd(header + 8, 1u);
+      cand[pos] = (int32_t)i;
+      reinterpret_cast<uint64_t*>(cand + cand_mask_offset(P.num_prims))[pos] = mask;
+    }
+    return;
+  }
+  if (near(rayo)) {
+    const uint32_t pos = atomicAdd(header + 8, 1u);
     cand[pos] = (int32_t)i;
   }
 }
@@ -197,80 +219,136 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
   return tn <= tf;
 }
 
-// FP64 depth of one event for the integrator: roots of |M (o + t d - mean)| = 1 with the record's M rows and the
-// primitive's mean (the same formula as tri-intersect.slang:268-283, evaluated in double so that depth
-// differences keep their precision far from the origin).
-__device__ __forceinline__ double event_depth_fp64(float4 RA, float4 RB, float4 RC, f3 mu, double ox, double oy,
-                                                   double oz, double dx, double dy, double dz, bool entry) {
-  const double vx = ox - (double)mu.x, vy = oy - (double)mu.y, vz = oz - (double)mu.z;
-  const double o0 = RA.x * vx + RA.y * vy + RA.z * vz, o1 = RB.x * vx + RB.y * vy + RB.z * vz,
-               o2 = RC.x * vx + RC.y * vy + RC.z * vz;
-  const double r0 = RA.x * dx + RA.y * dy + RA.z * dz, r1 = RB.x * dx + RB.y * dy + RB.z * dz,
-               r2 = RC.x * dx + RC.y * dy + RC.z * dz;
-  const double a = r0 * r0 + r1 * r1 + r2 * r2;
-  const double bp = -(o0 * r0 + o1 * r1 + o2 * r2);
-  const double ba = bp / a;
-  const double l0 = o0 + ba * r0, l1 = o1 + ba * r1, l2 = o2 + ba * r2;
-  const double h = fmax(a * (1.0 - (l0 * l0 + l1 * l1 + l2 * l2)), 0.0);
-  const double c = o0 * o0 + o1 * o1 + o2 * o2 - 1.0;
-  const double q = bp + (bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0)) * sqrt(h);
-  const double x0 = c / q, x1 = q / a;
-  return entry ? fmin(x0, x1) : fmax(x0, x1);
+// Grazing incidence. The record rows M (fp32 products of the rotation and 1/scale) and b = -M mean carry a relative
+// error of a few float32 ulps, so 1 - |l|^2 evaluated from them is off by up to about eps32 * |M o + b| terms, i.e.
+// eps32 * (|ocn| + |b|). Where 1 - |l|^2 lies within a band of kGrazeUlps such units of zero, whether the ray meets
+// the ellipsoid and the length of its chord are taken from the raw parameters in FP64, as the FP64 oracle does; the
+// chord of a grazing hit is short but its derivatives grow as 1 / sqrt(1 - |l|^2), so both must match.
+constexpr float kGrazeUlps = 64.f * 5.9604645e-8f;
+constexpr float kGraze2 = kGrazeUlps * kGrazeUlps;
+
+// The ray in FP64 (ray_origin_fp64, ray_dir_fp64).
+__device__ __forceinline__ void ray_fp64(const FwdArgs& A, int64_t r, d3& o, d3& d) {
+  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
+  o = ray_origin_fp64(ro, rdraw, A.tmin);
+  d = ray_dir_fp64(rdraw);
 }
 
-// Per-ray state of the forward pass: the ray in fp32 (traversal, leaf tests), the integrator state and the
-// (t, key) cursor. The FP64 ray of the event depths is rebuilt from the inputs for each batch of events, so it
-// occupies no registers during the traversal.
+// Roots of primitive s in FP64 from its raw parameters (raw_roots_d).
+__device__ __forceinline__ bool raw_roots_fp64(const FwdArgs& A, uint32_t s, d3 o, d3 d, double& t0, double& t1) {
+  return raw_roots_d(o, d, reinterpret_cast<const float4*>(A.quats)[s], ld3(A.scales + 3 * (int64_t)s),
+                     ld3(A.means + 3 * (int64_t)s), t0, t1);
+}
+
+// Per-ray state of the traversal: the ray in fp32 (box and leaf tests), the (t, key) cursor and what decides which
+// events a traversal gathers. The integrator's state (IntState) is kept in the forward workspace between batches, so
+// that the traversal loops carry none of it in registers.
 struct RayState {
-  int32_t r;  // ray index
-  int32_t e;  // column of the ray's events in tri: the ray index, or its processing slot (events by slot)
-  f3 o, d, inv, oi;
-  double D0, D1, D2, D3, t_state, logT;
-  float C0, C1, C2, csx, csy, dpx, dpy;
-  float cur_t;
-  uint64_t cursor;
-  int iter;
-  float bound;  // events beyond this depth are left for a later (plain) traversal; +inf once plain
-  // depth of the last event of the last completed 16-event round of the reference (the start of its current
-  // trace): an exit beyond tmax is gathered only when its entry lies after it (see leaf_events)
+  int32_t r;     // ray index
+  int32_t e;     // column of the ray's events in tri: the ray index, or its processing slot (events by slot)
+  int32_t slot;  // processing slot (the ray's IntState)
+  uint64_t cursor;  // (t, key) of the last event taken from a buffer (integrated, skipped or held); t = cur_t(S)
+  float bound;      // events beyond this depth are left for a later (plain) traversal; +inf once plain
+  // float32 depth of the last event of the last completed 16-event round of the reference (the start of its current
+  // trace), lowered by the gathering band: exits beyond tmax are gathered when their entry lies after it
   float tref;
+  bool live;  // the ray has not terminated
 };
 
+// The ray in float32 for the box and leaf tests, rebuilt from the inputs at the start of each traversal so that it
+// occupies no registers while the integrator runs: origin shifted by tmin along the raw direction
+// (shaders.slang:178-179), unit direction, its reciprocal and origin * reciprocal (slab distances as
+// fma(plane, inv, -o * inv)).
+struct RayGeo {
+  f3 o, d, inv, oi;
+};
+__device__ __forceinline__ RayGeo ray_geo(const FwdArgs& A, int64_t r) {
+  RayGeo G;
+  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
+  G.d = l2n(rdraw);
+  G.o = ro + A.tmin * rdraw;
+  G.inv = mk3(fabsf(G.d.x) > 1e-20f ? 1.f / G.d.x : copysignf(1e20f, G.d.x),
+              fabsf(G.d.y) > 1e-20f ? 1.f / G.d.y : copysignf(1e20f, G.d.y),
+              fabsf(G.d.z) > 1e-20f ? 1.f / G.d.z : copysignf(1e20f, G.d.z));
+  G.oi = mk3(G.o.x * G.inv.x, G.o.y * G.inv.y, G.o.z * G.inv.z);
+  return G;
+}
+
+// Integrator state (the reference's SplineState, structs.h:41-54, with FP64 density/colour accumulators and
+// log-transmittance), one 112-byte record per processing slot in the forward workspace, moved as seven 16-byte
+// words from one base address: (D0, D1) (D2, D3) (t_state, logT) (ts_state, logT_s) (tref64, hold)
+// (C0, C1, C2, csx) (csy, dpx, dpy, iter).
+struct IntState {
+  double D0, D1, D2, D3, t_state, logT;
+  // the same accumulation with the backward's safe roots (rotated direction l2-normalized, safe_roots_d): the
+  // log-transmittance the adjoint starts from and unwinds event by event with exactly these depths
+  double ts_state, logT_s;
+  double tref64;  // FP64 depth of the last event of the last completed 16-event round
+  float C0, C1, C2, csx, csy, dpx, dpy;
+  int32_t iter;
+  uint64_t hold;  // an event taken from the previous buffer but not yet integrated (0: none)
+};
+constexpr int64_t kIstBytes = 112;
+
+__device__ __forceinline__ void ist_load(const FwdArgs& A, int64_t slot, IntState& I) {
+  const double2* d = reinterpret_cast<const double2*>(A.ist + kIstBytes * slot);
+  const double2 a = d[0], b = d[1], c = d[2], e = d[3], g = d[4];
+  const float4 f = reinterpret_cast<const float4*>(d)[5], h = reinterpret_cast<const float4*>(d)[6];
+  I.D0 = a.x; I.D1 = a.y; I.D2 = b.x; I.D3 = b.y; I.t_state = c.x; I.logT = c.y; I.ts_state = e.x; I.logT_s = e.y;
+  I.tref64 = g.x; I.hold = (uint64_t)__double_as_longlong(g.y);
+  I.C0 = f.x; I.C1 = f.y; I.C2 = f.z; I.csx = f.w; I.csy = h.x; I.dpx = h.y; I.dpy = h.z; I.iter = __float_as_int(h.w);
+}
+__device__ __forceinline__ void ist_store(const FwdArgs& A, int64_t slot, const IntState& I) {
+  double2* d = reinterpret_cast<double2*>(A.ist + kIstBytes * slot);
+  d[0] = make_double2(I.D0, I.D1);
+  d[1] = make_double2(I.D2, I.D3);
+  d[2] = make_double2(I.t_state, I.logT);
+  d[3] = make_double2(I.ts_state, I.logT_s);
+  d[4] = make_double2(I.tref64, __longlong_as_double((long long)I.hold));
+  reinterpret_cast<float4*>(d)[5] = make_float4(I.C0, I.C1, I.C2, I.csx);
+  reinterpret_cast<float4*>(d)[6] = make_float4(I.csy, I.dpx, I.dpy, __int_as_float(I.iter));
+}
+
+__device__ __forceinline__ bool int_alive(const FwdArgs& A, const IntState& I) {
+  return I.logT < (double)kLogCutoff && I.iter < A.max_iters;
+}
+
+// Gathering band around tmax and the round boundary: the float32 leaf depths gather every event whose eligibility
+// could hold in FP64 (their error is far below this band), and the integrator decides it from the FP64 depths.
+constexpr float kBand = 1e-4f;
+__device__ __forceinline__ float band_lo(float t) { return t - kBand * (fabsf(t) + 1.f); }
+__device__ __forceinline__ float band_hi(float t) { return t + kBand * (fabsf(t) + 1.f); }
+
+// depth of the cursor (the float32 depth of the last event taken)
+__device__ __forceinline__ float cur_t(const RayState& S) { return __uint_as_float((uint32_t)(S.cursor >> 32)); }
+
 // A lane of the per-wave walk with no ray: it stays in the wave (the wave's stack keeps entry i in lane i, and a
-// lane that has left the wave would drop the entries written to it) and is never alive (iter = max_iters), so it
-// never enters a box.
+// lane that has left the wave would drop the entries written to it) and is never live, so it never enters a box.
 __device__ __forceinline__ void ray_idle(const FwdArgs& A, RayState& S) {
   S.r = 0;
   S.e = 0;
-  S.o = S.d = S.oi = mk3(0.f, 0.f, 0.f);
-  S.inv = mk3(1.f, 1.f, 1.f);
-  S.D0 = S.D1 = S.D2 = S.D3 = 0.0;
-  S.t_state = S.logT = 0.0;
-  S.C0 = S.C1 = S.C2 = S.csx = S.csy = S.dpx = S.dpy = 0.f;
-  S.cur_t = 0.f;
+  S.slot = 0;
   S.cursor = 0;
-  S.iter = A.max_iters;
   S.bound = INFINITY;
   S.tref = 0.f;
+  S.live = false;
 }
 
 template <bool kViews>
 __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t slot, RayState& S) {
   S.r = (int32_t)r;
   S.e = (int32_t)(A.events_by_slot ? slot : r);
-  const f3 ro = ld3(A.rayo + 3 * r), rdraw = ld3(A.rayd + 3 * r);
-  S.d = l2n(rdraw);
-  S.o = ro + A.tmin * rdraw;  // shaders.slang:178-179: origin shifted along the raw direction
-  S.inv = mk3(fabsf(S.d.x) > 1e-20f ? 1.f / S.d.x : copysignf(1e20f, S.d.x),
-              fabsf(S.d.y) > 1e-20f ? 1.f / S.d.y : copysignf(1e20f, S.d.y),
-              fabsf(S.d.z) > 1e-20f ? 1.f / S.d.z : copysignf(1e20f, S.d.z));
-  S.oi = mk3(S.o.x * S.inv.x, S.o.y * S.inv.y, S.o.z * S.inv.z);  // slab distances as fma(plane, inv, -o*inv)
+  S.slot = (int32_t)slot;
+  const f3 ro = ld3(A.rayo + 3 * r);
   // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
   double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
   const int32_t ncand = (int32_t)A.header[8];
-  const f3 p = ro + A.tmin * S.d;
+  const f3 p = ro + A.tmin * l2n(ld3(A.rayd + 3 * r));
+  const int32_t view = kViews ? A.view_index[r] : 0;
   for (int32_t j = 0; j < ncand; ++j) {
     const int64_t i = A.cand[j];
+    // several views: only the candidates of this ray's view
+    if (kViews && ((reinterpret_cast<const uint64_t*>(A.cand + cand_mask_offset(A.n))[j] >> view) & 1u) == 0) continue;
     float4 q = reinterpret_cast<const float4*>(A.quats)[i];
     float qn = 1.f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     q = make_float4(q.x * qn, q.y * qn, q.z * qn, q.w * qn);
@@ -281,7 +359,7 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
       float den = A.dens[i];
       float c0, c1, c2;
       if (A.color_mode == EVER_COLOR_SH) {
-        const float4 cr = kViews ? A.vcol[(int64_t)A.view_index[r] * A.n + i] : A.colr[i];
+        const float4 cr = kViews ? A.vcol[(int64_t)view * A.n + i] : A.colr[i];
         c0 = cr.y; c1 = cr.z; c2 = cr.w;
       } else {  // the reference reads the features at flat index prim*3 + c (initialize_density.cu:140-142)
         c0 = A.feat[3 * i + 0] * kShC0 + 0.5f;
@@ -295,29 +373,26 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
     }
   }
   reinterpret_cast<float4*>(A.init_drgb)[r] = make_float4((float)D0, (float)D1, (float)D2, (float)D3);
-  S.D0 = D0;
-  S.D1 = D1;
-  S.D2 = D2;
-  S.D3 = D3;
-  S.C0 = S.C1 = S.C2 = S.csx = S.csy = S.dpx = S.dpy = 0.f;
-  S.t_state = 0.0;
-  S.logT = 0.0;
+  IntState I;
+  I.D0 = D0;
+  I.D1 = D1;
+  I.D2 = D2;
+  I.D3 = D3;
+  I.t_state = I.logT = I.ts_state = I.logT_s = I.tref64 = 0.0;
+  I.C0 = I.C1 = I.C2 = I.csx = I.csy = I.dpx = I.dpy = 0.f;
+  I.iter = 0;
+  I.hold = 0;
+  ist_store(A, slot, I);
   // cursor over (t, key): initially nothing at t == 0 (the reference requires t > t_cur = 0)
-  S.cur_t = 0.f;
   S.cursor = pack_event(0.f, 0xffffffffu);
-  S.iter = 0;
   S.bound = INFINITY;
-  S.tref = 0.f;
-}
-
-
-__device__ __forceinline__ bool ray_alive(const FwdArgs& A, const RayState& S) {
-  return S.logT < (double)kLogCutoff && S.iter < A.max_iters;
+  S.tref = band_lo(0.f);
+  S.live = A.max_iters > 0;
 }
 
 // Depth at which the ray leaves the union of the root record's child boxes (+inf when the root is a leaf): a
 // proposal beyond it leaves no event for a later traversal.
-__device__ __forceinline__ float scene_exit(const FwdArgs& A, const RayState& S) {
+__device__ __forceinline__ float scene_exit(const FwdArgs& A, const RayGeo& S) {
   if (A.header[9] <= 1) return INFINITY;
   const float4* nd = A.nodes4;
   const float4 cf = nd[6];
@@ -343,9 +418,6 @@ __device__ __forceinline__ void ray_bound(const FwdArgs& A, RayState& S) {
   S.bound = b >= 0.f ? b : INFINITY;
 }
 
-// Leaf test of one primitive: ellipsoid roots (tri-intersect.slang:268-283) in the record's frame, then the
-// intersection program (shaders.slang:96-106) and the any-hit insertion of entry and exit (:138-159), with the
-// (t, key) cursor in place of t > t_cur.
 // The roots are written with explicit fused multiply-adds and no other contraction, so both tree walks (records
 // in vector or in scalar registers) produce bit-identical depths and therefore the same event order.
 __device__ __forceinline__ float dot3_fma(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
@@ -354,12 +426,14 @@ __device__ __forceinline__ float dot3_fma(f3 a, f3 b) { return fmaf(a.z, b.z, fm
 // intersection program (shaders.slang:96-106) and the any-hit choice of which events enter the k-buffer (:138-159),
 // with the (t, key) cursor in place of t > t_cur. Sets e (entry) and x (exit) to the events to insert, kNoEvent
 // where none. Entry and exit go in together when the entry lies after the cursor (and within tmax); otherwise the
-// exit alone when it lies after the cursor and within tmax.
+// exit alone when it lies after the cursor and within tmax, or beyond tmax with its entry in the current round.
+// The float32 depths decide within a band around tmax and the round's start: the integrator takes the final
+// decision from the FP64 depths (integrate_seq).
 template <bool kBounded>
-__device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S, float4 RA, float4 RB, float4 RC,
-                                            uint32_t s, uint64_t& e, uint64_t& x) {
+__device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S, const RayGeo& G, float4 RA,
+                                            float4 RB, float4 RC, uint32_t s, uint64_t& e, uint64_t& x) {
 #pragma clang fp contract(off)
-  const f3 o = S.o, d = S.d;
+  const f3 o = G.o, d = G.d;
   const f3 ocn = mk3(fmaf(RA.z, o.z, fmaf(RA.y, o.y, fmaf(RA.x, o.x, RA.w))),
                      fmaf(RB.z, o.z, fmaf(RB.y, o.y, fmaf(RB.x, o.x, RB.w))),
                      fmaf(RC.z, o.z, fmaf(RC.y, o.y, fmaf(RC.x, o.x, RC.w))));
@@ -373,19 +447,25 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
   const float ia = __builtin_amdgcn_rcpf(aa);
   const float ba = bp * ia;
   const f3 l = mk3(fmaf(ba, rdn.x, ocn.x), fmaf(ba, rdn.y, ocn.y), fmaf(ba, rdn.z, ocn.z));
-  const float h = aa * (1.f - dot3_fma(l, l));
+  const float omll = 1.f - dot3_fma(l, l);
+  const float h = aa * omll;
   const float cc = dot3_fma(ocn, ocn) - 1.f;
   const float qq = fmaf(sgnf(bp), __builtin_amdgcn_sqrtf(fmaxf(h, 0.f)), bp);
   const float r0 = cc * __builtin_amdgcn_rcpf(qq), r1 = qq * ia;
   const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
   const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
-  const bool hit = h >= 0.f && !(t1 < S.cur_t);
-  const bool with_entry = t0 >= S.cur_t && ve > S.cursor;
-  const bool ins_e = hit && with_entry && t0 <= A.tmax;
+  // a miss within the grazing band still enters the buffer (with the roots of 1 - |l|^2 clamped at 0): the
+  // integrator decides such events from the raw parameters in FP64 and skips the pair if the ray misses
+  const float bb = fmaf(RC.w, RC.w, fmaf(RB.w, RB.w, RA.w * RA.w));
+  const bool meets = h >= 0.f || omll * omll < kGraze2 * (cc + 1.f + 4.f * bb);
+  const bool hit = meets && !(t1 < cur_t(S));
+  const bool with_entry = t0 >= cur_t(S) && ve > S.cursor;
+  const float tmax = A.tmax_hi;
+  const bool ins_e = hit && with_entry && t0 <= tmax;
   // an exit beyond tmax enters only with its entry: in the reference's trace of the current round (from tref) the
   // entry is reported with the exit as its attribute; an entry before tref was reported in an earlier round
   const bool ins_x =
-      hit && (with_entry ? t0 <= A.tmax : (vx > S.cursor && (t1 <= A.tmax || (t0 > S.tref && t0 <= A.tmax))));
+      hit && (with_entry ? t0 <= tmax : (vx > S.cursor && (t1 <= tmax || (t0 > S.tref && t0 <= tmax))));
   if (kBounded) {
     // under a proposal only events up to it are gathered, so that every gathered event precedes every culled one
     e = ins_e && t0 <= S.bound ? ve : kNoEvent;
@@ -396,80 +476,162 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
   }
 }
 
-// Integrates the gathered events head_at(0), head_at(1), ... in order (shaders.slang:210-226); returns true when
-// the buffer held fewer than K events, i.e. the ray has no event left.
-template <int K, bool kViews, typename Head>
+// The control point of an integrated event (get_ctrl_pt, shaders.slang:65-82): +-density * (1, colour).
+template <bool kViews>
+__device__ __forceinline__ void write_last_dirac(const FwdArgs& A, const RayState& S, uint32_t key) {
+  const uint32_t s = key >> 1;
+  const float4 Dr = kViews ? A.vcol[(int64_t)A.view_index[S.r] * A.n + s] : A.rec[4 * (int64_t)s + 3];
+  const float h = (key & 1u) == 0u ? Dr.x : -Dr.x;
+  reinterpret_cast<float4*>(A.last_dirac)[S.r] = make_float4(h, h * Dr.y, h * Dr.z, h * Dr.w);
+}
+
+// Integrates the gathered events head_at(0), head_at(1), ... (each fetched once, in buffer order) as the reference's
+// raygen loop does (shaders.slang:210-226); returns true when the buffer held fewer than K events, i.e. no event
+// is left up to the bound of this traversal.
+// Depths, order and eligibility are those of FP64, from the raw parameters, as in the oracle:
+// - the buffer is ordered by the float32 depths of the leaf test; one event is held back, across buffers too
+//   (IntState.hold), so that where two neighbours tie within float32 resolution and their FP64 depths order them
+//   the other way, the later-fetched one is integrated first (the clipped step max(t - t_prev, 0) makes the
+//   gradients depend on that order);
+// - a grazing miss in FP64 is skipped; an entry beyond tmax is skipped; an exit beyond tmax counts only while its
+//   entry lies after the last event of the reference's last completed 16-event round (BUFFER_SIZE,
+//   shaders.slang:18), i.e. when the reference's trace of the current round reports it with its entry, which
+//   makes results independent of K.
+template <int K, bool kBounded, bool kViews, typename Head>
 __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Head head_at) {
-  // the ray in FP64: origin shifted by tmin along the raw direction, unit direction
-  const f3 ro = ld3(A.rayo + 3 * (int64_t)S.r), rdraw = ld3(A.rayd + 3 * (int64_t)S.r);
-  const double odx = (double)ro.x + (double)A.tmin * rdraw.x, ody = (double)ro.y + (double)A.tmin * rdraw.y,
-               odz = (double)ro.z + (double)A.tmin * rdraw.z;
-  double ddx = rdraw.x, ddy = rdraw.y, ddz = rdraw.z;
-  {
-    const double nn = 1.0 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-    ddx *= nn;
-    ddy *= nn;
-    ddz *= nn;
-  }
-  // Results equal the reference's 16-event rounds (BUFFER_SIZE, shaders.slang:18) for any K: events up to tmax
-  // do not depend on the rounds, and an exit beyond tmax is eligible when its entry lies after the start of the
-  // reference's current round (tref). Once this batch has crossed a round boundary, exits beyond tmax in it were
-  // gathered under the earlier tref, so the batch stops before them and the next traversal gathers again.
-  bool crossed = false;
-  for (int k = 0; k < K; ++k) {
-    const uint64_t head = head_at(k);
-    const float tk = __uint_as_float((uint32_t)(head >> 32));
-    if (tk > kEndT) return true;
-    if (crossed && tk > A.tmax) return false;
+  IntState I;
+  ist_load(A, S.slot, I);
+  d3 od, dd;  // the ray in FP64
+  ray_fp64(A, S.r, od, dd);
+  const f3 rdraw = ld3(A.rayd + 3 * (int64_t)S.r);
+  uint32_t last = 0xffffffffu;  // key of the last event integrated in this batch
+  uint64_t ph = 0;              // the held-back event, its FP64 depth and its round condition (below)
+  double pt = 0.0, pt0 = 0.0;
+  bool held = false;
+  uint64_t carried = I.hold;  // the previous buffer's held event is fetched first
+  I.hold = 0;
+  bool end = false;  // the buffer's end marker was fetched
+  int k = 0;         // next buffer entry to fetch
+  bool ret = false;
+  while (true) {
+    // fetch the next eligible event: its FP64 depth, and for an exit beyond tmax the depth of its entry, which
+    // must lie in the current round when the exit is integrated (-inf: no condition)
+    uint64_t nh = 0;
+    double nt = 0.0, nt0 = 0.0;
+    bool got = false;
+    while (carried != 0 || (!end && k < K)) {
+      uint64_t head = carried;
+      carried = 0;
+      if (head == 0) {
+        head = head_at(k++);
+        if (__uint_as_float((uint32_t)(head >> 32)) > kEndT) {
+          end = true;
+          break;
+        }
+      }
+      S.cursor = S.cursor > head ? S.cursor : head;
+      double t0, t1;
+      if (!raw_roots_fp64(A, (uint32_t)head >> 1, od, dd, t0, t1)) continue;  // a grazing miss
+      if (t0 > (double)A.tmax) continue;  // an entry beyond tmax, or an exit whose entry is beyond it
+      const bool entry = (head & 1u) == 0u;
+      nt = entry ? t0 : t1;
+      nt0 = (entry || t1 <= (double)A.tmax) ? -INFINITY : t0;
+      nh = head;
+      got = true;
+      break;
+    }
+    if (!got) {
+      if (!held) {
+        ret = end;
+        break;
+      }
+      // a full buffer, or the end of the events up to the bound of a ray that continues beyond its proposal: the
+      // held event waits for the next traversal, whose first event may precede it in FP64
+      if (!end || (kBounded && S.bound != INFINITY && S.bound < scene_exit(A, ray_geo(A, S.r)))) {
+        I.hold = ph;
+        ret = end;
+        break;
+      }
+    } else if (!held) {
+      ph = nh;
+      pt = nt;
+      pt0 = nt0;
+      held = true;
+      continue;
+    }
+    // the earlier of the held-back and the fetched event in (FP64 depth, key) order; the other stays held
+    const bool take_new = got && (nt < pt || (nt == pt && (uint32_t)nh < (uint32_t)ph));
+    const uint64_t head = take_new ? nh : ph;
+    const double t = take_new ? nt : pt;
+    const double t0r = take_new ? nt0 : pt0;
+    if (!take_new) {
+      ph = nh;
+      pt = nt;
+      pt0 = nt0;
+      held = got;
+    }
+    if (t0r != -INFINITY && !(t0r > I.tref64)) continue;  // its entry lies in an earlier round
     const uint32_t key = (uint32_t)head;
     const uint32_t s = key >> 1;
     const bool entry = (key & 1u) == 0u;
-    const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
     // (density, colour): the record's, or this ray's view's when several views share the batch
     const float4 Dr = kViews ? A.vcol[(int64_t)A.view_index[S.r] * A.n + s] : A.rec[4 * (int64_t)s + 3];
     const uint32_t orig = s;  // records are in original primitive order
-    const f3 mu = ld3(A.means + 3 * (int64_t)orig);
-    const double t = event_depth_fp64(RA, RB, RC, mu, odx, ody, odz, ddx, ddy, ddz, entry);
     const float hgt = entry ? Dr.x : -Dr.x;
     // update (spline-machine.slang:103-138)
-    const float dt = fmaxf((float)(t - S.t_state), 0.f);
-    const float Df = (float)S.D0;
+    const float dt = fmaxf((float)(t - I.t_state), 0.f);
+    const float Df = (float)I.D0;
     const float area = fmaxf(Df * dt, 0.f);
-    const float rn0 = safe_div((float)S.D1, Df), rn1 = safe_div((float)S.D2, Df), rn2 = safe_div((float)S.D3, Df);
-    const float e2 = expf(-(float)S.logT);
-    const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
-    S.logT = fmax((double)area + S.logT, 0.0);
-    S.C0 += w * rn0;
-    S.C1 += w * rn1;
-    S.C2 += w * rn2;
-    const float m = ladder((float)((t + S.t_state) * (0.5 * kPreMulti)));
-    S.dpx += 2.f * w * m * S.csx;
-    S.dpy += 2.f * w * S.csy;
-    S.csx += w;
-    S.csy += w * m;
-    S.D0 += (double)hgt;
-    S.D1 += (double)(hgt * Dr.y);
-    S.D2 += (double)(hgt * Dr.z);
-    S.D3 += (double)(hgt * Dr.w);
-    S.t_state = t;
-    S.cur_t = tk;
-    S.cursor = head;
-    if (A.tri) A.tri[S.e + (int64_t)S.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
-    if (A.touch) atomicAdd(A.touch + orig, 1);
-    ++S.iter;
-    if ((S.iter & 15) == 0) {
-      S.tref = tk;
-      crossed = true;
+    {
+      // the adjoint's depth of this event and its area, as ever_backward evaluates them
+      const float4 q = reinterpret_cast<const float4*>(A.quats)[s];
+      const double ts = safe_depth(od - tod(ld3(A.means + 3 * (int64_t)s)), rdraw, ld3(A.scales + 3 * (int64_t)s),
+                                   q, entry ? 1 : 0);
+      const float dts = fmaxf((float)(ts - I.ts_state), 0.f);
+      I.logT_s = fmax((double)fmaxf(Df * dts, 0.f) + I.logT_s, 0.0);
+      I.ts_state = ts;
     }
-    if (!ray_alive(A, S)) return false;
+    const float rn0 = safe_div((float)I.D1, Df), rn1 = safe_div((float)I.D2, Df), rn2 = safe_div((float)I.D3, Df);
+    const float e2 = expf(-(float)I.logT);
+    const float w = clipf(-expm1f(-area) * e2, 0.f, 1.f);
+    I.logT = fmax((double)area + I.logT, 0.0);
+    I.C0 += w * rn0;
+    I.C1 += w * rn1;
+    I.C2 += w * rn2;
+    const float m = ladder((float)((t + I.t_state) * (0.5 * kPreMulti)));
+    I.dpx += 2.f * w * m * I.csx;
+    I.dpy += 2.f * w * I.csy;
+    I.csx += w;
+    I.csy += w * m;
+    I.D0 += (double)hgt;
+    I.D1 += (double)(hgt * Dr.y);
+    I.D2 += (double)(hgt * Dr.z);
+    I.D3 += (double)(hgt * Dr.w);
+    I.t_state = t;
+    last = key;
+    if (A.tri) A.tri[S.e + (int64_t)I.iter * A.nr] = (int32_t)(2u * orig + (entry ? 1u : 0u));
+    if (A.touch) atomicAdd(A.touch + orig, 1);
+    ++I.iter;
+    if ((I.iter & 15) == 0) {
+      I.tref64 = t;
+      S.tref = band_lo((float)t);
+    }
+    if (!int_alive(A, I)) {
+      ret = false;
+      break;
+    }
   }
-  return false;
+  ist_store(A, S.slot, I);
+  S.live = int_alive(A, I);
+  // the caller's last integrated control point (shaders.slang:229-232)
+  if (A.last_dirac && last != 0xffffffffu) write_last_dirac<kViews>(A, S, last);
+  return ret;
 }
 
-template <int K, bool kViews>
+template <int K, bool kBounded, bool kViews>
 __device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, uint64_t (&bk)[K]) {
   // the head of the buffer is consumed and the rest shifted down, so the loop body exists once
-  return integrate_seq<K, kViews>(A, S, [&](int) {
+  return integrate_seq<K, kBounded, kViews>(A, S, [&](int) {
     const uint64_t head = bk[0];
 #pragma unroll
     for (int j = 0; j < K - 1; ++j) bk[j] = bk[j + 1];
@@ -480,37 +642,31 @@ __device__ __forceinline__ bool integrate_events(const FwdArgs& A, RayState& S, 
 // extract_color (spline-machine.slang:146-153) + distortion assembly (fast_ellipsoid_splinetracer.py:81-85)
 __device__ __forceinline__ void ray_write(const FwdArgs& A, const RayState& S) {
   const int64_t r = S.r;
+  IntState I;
+  ist_load(A, S.slot, I);
   float* oc = A.color + 5 * r;
-  oc[0] = S.C0;
-  oc[1] = S.C1;
-  oc[2] = S.C2;
-  oc[3] = (float)(-expm1(-S.logT));
-  oc[4] = S.dpx - S.dpy;
+  oc[0] = I.C0;
+  oc[1] = I.C1;
+  oc[2] = I.C2;
+  oc[3] = (float)(-expm1(-I.logT));
+  oc[4] = I.dpx - I.dpy;
   float4* os = reinterpret_cast<float4*>(A.states + 16 * r);
-  os[0] = make_float4(S.dpx, S.dpy, S.csx, S.csy);
-  os[1] = make_float4(0.f, 0.f, 0.f, (float)S.t_state);
-  os[2] = make_float4((float)S.D0, (float)S.D1, (float)S.D2, (float)S.D3);
-  os[3] = make_float4((float)S.logT, S.C0, S.C1, S.C2);
-  A.iters[r] = S.iter;
-  double* s64 = A.st64 + 6 * r;
-  s64[0] = S.D0;
-  s64[1] = S.D1;
-  s64[2] = S.D2;
-  s64[3] = S.D3;
-  s64[4] = S.logT;
-  s64[5] = S.t_state;
-  if (A.last_dirac) {
-    // the last integrated event's control point (get_ctrl_pt, shaders.slang:65-82), recovered from the cursor
-    float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (S.iter > 0) {
-      const uint32_t key = (uint32_t)S.cursor;
-      const uint32_t s = key >> 1;
-      const float4 Dr = A.vcol ? A.vcol[(int64_t)A.view_index[r] * A.n + s] : A.rec[4 * (int64_t)s + 3];
-      const float h = (key & 1u) == 0u ? Dr.x : -Dr.x;
-      dv = make_float4(h, h * Dr.y, h * Dr.z, h * Dr.w);
-    }
-    reinterpret_cast<float4*>(A.last_dirac)[r] = dv;
-  }
+  os[0] = make_float4(I.dpx, I.dpy, I.csx, I.csy);
+  os[1] = make_float4(0.f, 0.f, 0.f, (float)I.t_state);
+  os[2] = make_float4((float)I.D0, (float)I.D1, (float)I.D2, (float)I.D3);
+  os[3] = make_float4((float)I.logT, I.C0, I.C1, I.C2);
+  A.iters[r] = I.iter;
+  double* s64 = A.st64 + 8 * r;
+  s64[0] = I.D0;
+  s64[1] = I.D1;
+  s64[2] = I.D2;
+  s64[3] = I.D3;
+  s64[4] = I.logT;
+  s64[5] = I.t_state;
+  s64[6] = I.logT_s;
+  s64[7] = I.ts_state;
+  // the last integrated control point is written by the integrator; a ray without events has none
+  if (A.last_dirac && I.iter == 0) reinterpret_cast<float4*>(A.last_dirac)[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Per-ray traversal: one lane walks the tree for its own ray with a short LDS stack (global spill beyond
@@ -525,18 +681,19 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
   RayState S;
   ray_setup<kViews>(A, A.order[slot], slot, S);
   if (kBounded) ray_bound(A, S);
-  const f3 inv = S.inv, oi = S.oi;
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
   int32_t* spill = A.spill + slot * kSpillDepth;
 
-  while (ray_alive(A, S)) {
+  while (S.live) {
+    const RayGeo G = ray_geo(A, S.r);
+    const f3 inv = G.inv, oi = G.oi;
     uint64_t bk[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
     // the lower culling depth is the cursor, held at tmax once the cursor has passed it: exits beyond tmax whose
     // entries lie before it may remain, and their boxes straddle tmax
-    const float cur_t = fminf(S.cur_t, A.tmax);
+    const float t_lo = fminf(cur_t(S), A.tmax_lo);
     // ---- traversal: gather the K nearest events after the cursor ----
     int sp = 0;
     int32_t node = root;
@@ -547,7 +704,8 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         // 4-wide record: slab test of up to four children, nearest first
         const float4* nd = A.nodes4 + 8 * (int64_t)node;
         const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cf = nd[6];
-        const float t_hi = fminf(kBounded ? fminf(A.tmax, S.bound) : A.tmax, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
+        const float t_hi =
+            fminf(kBounded ? fminf(A.tmax_hi, S.bound) : A.tmax_hi, __uint_as_float((uint32_t)(bk[K - 1] >> 32)));
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
         const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
@@ -564,7 +722,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
           const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
           const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
           const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
-          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), cur_t));
+          const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
           const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
           // unused slots (kEmptyChild) never hit, whatever their stored box
           const bool h = tn <= tf && used[k];
@@ -587,7 +745,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
           };
 #pragma unroll
           for (int k = 3; k >= 1; --k) {
-            if (k < nhit && sp < kLdsDepth + kSpillDepth) {  // bounded by 3 x (LBVH depth / 2)
+            if (k < nhit) {  // sp < kStackBound <= kLdsDepth + kSpillDepth (tree bound, ever_internal.hpp)
               const int32_t ck = child(key[k]);
               if (sp < kLdsDepth) stk[sp * kFwdBlock + lane] = ck;
               else spill[sp - kLdsDepth] = ck;
@@ -601,7 +759,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
         const uint32_t s = (uint32_t)(~node);
         const float4 RA = A.rec[4 * (int64_t)s + 0], RB = A.rec[4 * (int64_t)s + 1], RC = A.rec[4 * (int64_t)s + 2];
         uint64_t e, x;
-        leaf_events<kBounded>(A, S, RA, RB, RC, s, e, x);
+        leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
         kbuf_insert<K>(bk, e);
         kbuf_insert<K>(bk, x);
       }
@@ -609,10 +767,10 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
       --sp;
       node = sp < kLdsDepth ? stk[sp * kFwdBlock + lane] : spill[sp - kLdsDepth];
     }
-    if (integrate_events<K, kViews>(A, S, bk)) {
+    if (integrate_events<K, kBounded, kViews>(A, S, bk)) {
       // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
       // else continue without the proposal
-      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) break;
+      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, ray_geo(A, S.r))) break;
       S.bound = INFINITY;
     }
   }
@@ -659,26 +817,30 @@ __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int 
 
 template <int K, bool kSigns, bool kBounded, bool kViews>
 __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int lane, uint32_t sg,
-                                           uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch) {
-  const f3 inv = S.inv, oi = S.oi;
+                                           uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch, int32_t* row2) {
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
   bool done = false;  // (a lane without a ray is never alive: ray_idle)
 
   while (true) {
-    const bool alive = !done && ray_alive(A, S);
+    const bool alive = !done && S.live;
     const uint64_t live = __ballot(alive);
     if (live == 0) break;
     const int leader = __builtin_ctzll(live);
+    const RayGeo G = ray_geo(A, S.r);
+    const f3 inv = G.inv, oi = G.oi;
 #pragma unroll
     for (int k = 0; k < K; ++k) ev[k][lane] = kNoEvent;
     uint64_t kmax = kNoEvent;
     // a finished lane never enters a box; the cursor is held at tmax once past it (exits beyond tmax whose entries
     // lie before it may remain, in boxes that straddle tmax)
-    float t_lo = alive ? fminf(S.cur_t, A.tmax) : INFINITY;
+    float t_lo = alive ? fminf(cur_t(S), A.tmax_lo) : INFINITY;
     // kept opaque: otherwise the select is folded into every box test below (two extra selects per child)
     asm volatile("" : "+v"(t_lo));
-    int32_t stk0 = 0, stk1 = 0;                     // wave stack: entry i in lane i % 64 of stk(i / 64)
+    // wave stack: entry i in lane i % 64 of row i / 64; rows 0 and 1 in VGPRs, row 2 in LDS (entries beyond 128,
+    // needed only by trees deeper than 42 records; 192 entries cover the bound of 180). Entry 128 + i is written
+    // and read back by lane i only.
+    int32_t stk0 = 0, stk1 = 0;
     int sp = 0;
     int32_t node = root;
     int64_t steps = 0;
@@ -693,7 +855,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
                      hy = ld_uniform4(nd, 3 - ey), lz = ld_uniform4(nd, 4 + ez), hz = ld_uniform4(nd, 5 - ez),
                      cf = ld_uniform4(nd, 6);
         float t_hi =
-            alive ? fminf(kBounded ? fminf(A.tmax, S.bound) : A.tmax, __uint_as_float((uint32_t)(kmax >> 32)))
+            alive ? fminf(kBounded ? fminf(A.tmax_hi, S.bound) : A.tmax_hi, __uint_as_float((uint32_t)(kmax >> 32)))
                   : -INFINITY;
         asm volatile("" : "+v"(t_hi));
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
@@ -746,10 +908,11 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
           };
 #pragma unroll
           for (int k = 3; k >= 1; --k) {
-            if (k < nhit) {  // depth of a 4-wide LBVH record tree <= 32, so sp < 3 * 32 + 1 <= 2 * 64
+            if (k < nhit) {  // sp < kStackBound <= 3 * 64 (tree bound, ever_internal.hpp)
               const int32_t ck = child(key[k]);
               if (sp < 64) stk0 = (lane == sp) ? ck : stk0;
-              else stk1 = (lane == sp - 64) ? ck : stk1;
+              else if (sp < 128) stk1 = (lane == sp - 64) ? ck : stk1;
+              else if (lane == sp - 128) row2[lane] = ck;
               ++sp;
               // bring the record into L2 before it is popped: an asynchronous copy into a scratch area of LDS
               // that nothing reads
@@ -769,18 +932,20 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4* rc = A.rec + 4 * (int64_t)s;
         const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
         uint64_t e = kNoEvent, x = kNoEvent;
-        if (alive) leaf_events<kBounded>(A, S, RA, RB, RC, s, e, x);
+        if (alive) leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
         if (__builtin_amdgcn_ballot_w64(e < kmax) != 0) kbuf_insert_lds<K>(ev, lane, e, kmax);
         if (__builtin_amdgcn_ballot_w64(x < kmax) != 0) kbuf_insert_lds<K>(ev, lane, x, kmax);
       }
       if (sp == 0) break;
       --sp;
-      node = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp) : __builtin_amdgcn_readlane(stk1, sp - 64);
+      if (sp < 64) node = __builtin_amdgcn_readlane(stk0, sp);
+      else if (sp < 128) node = __builtin_amdgcn_readlane(stk1, sp - 64);
+      else node = __builtin_amdgcn_readlane(row2[lane], sp - 128);  // lane sp - 128 reads back its own store
     }
-    if (alive && integrate_seq<K, kViews>(A, S, [&](int k) { return ev[k][lane]; })) {
+    if (alive && integrate_seq<K, kBounded, kViews>(A, S, [&](int k) { return ev[k][lane]; })) {
       // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
       // else continue without the proposal
-      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, S)) done = true;
+      if (!kBounded || S.bound == INFINITY || S.bound >= scene_exit(A, ray_geo(A, S.r))) done = true;
       else S.bound = INFINITY;
     }
   }
@@ -792,6 +957,7 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
   const int lane = threadIdx.x;
   __shared__ uint64_t ev[K][kWaveBlock];
   __shared__ uint32_t pf_scratch[kWaveBlock];
+  __shared__ int32_t row2[kWaveBlock];  // third row of the wave's traversal stack
   // the 64 rays of processing slots [base, base + 64); lanes past the batch stay in the wave without a ray
   // (ray_idle), so that all 64 lanes hold the wave's stack
   auto trace_tile = [&](int64_t base) {
@@ -806,11 +972,12 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
       ray_idle(A, S);
     }
     // lane 0 always has a ray; rayless lanes take its signs so they do not break the wave's sign uniformity
-    uint32_t sg = (signbit(S.inv.x) ? 1u : 0u) | (signbit(S.inv.y) ? 2u : 0u) | (signbit(S.inv.z) ? 4u : 0u);
+    const f3 inv0 = ray_geo(A, S.r).inv;
+    uint32_t sg = (signbit(inv0.x) ? 1u : 0u) | (signbit(inv0.y) ? 2u : 0u) | (signbit(inv0.z) ? 4u : 0u);
     const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
     if (!valid) sg = sg0;
-    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
-    else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch);
+    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch, row2);
+    else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch, row2);
     if (base + lane < A.nr) ray_write(A, S);
   };
   if (!kPersistent) {
@@ -871,7 +1038,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
     hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, P, R.origins, R.tmin, cand,
                        header);
   // processing order
-  ForwardLayout FL = forward_layout(R.num_rays);
+  ForwardLayout FL = forward_layout(R.num_rays, per_ray_walk(R.num_rays, R.traversal));
   uint32_t* k0 = at<uint32_t>(O.workspace, FL.keys0);
   uint32_t* v0 = at<uint32_t>(O.workspace, FL.vals0);
   if (R.num_rays >= kMinRaysToSort && P.num_prims > 0) {
@@ -902,6 +1069,8 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.nr = R.num_rays;
   A.tmin = R.tmin;
   A.tmax = R.tmax;
+  A.tmax_lo = R.tmax - 1e-4f * (fabsf(R.tmax) + 1.f);
+  A.tmax_hi = R.tmax + 1e-4f * (fabsf(R.tmax) + 1.f);
   A.max_iters = R.max_iters;
   A.color = O.color;
   A.states = O.states;
@@ -913,6 +1082,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   A.st64 = O.ray_state64;
   A.last_dirac = O.last_dirac;
   A.spill = at<int32_t>(O.workspace, FL.spill);
+  A.ist = at<char>(O.workspace, FL.ist);
   A.order = v0;
   A.proposal = R.proposal;
   A.vcol = P.num_views > 1 ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
@@ -924,8 +1094,7 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
     // no primitive: every ray sees no event; run the kernel with an empty loop by setting max_iters = 0
     A.max_iters = 0;
   }
-  const bool sorted = R.num_rays >= kMinRaysToSort && P.num_prims > 0;
-  const bool wave = R.traversal == EVER_TRAVERSAL_WAVE || (R.traversal == EVER_TRAVERSAL_AUTO && sorted);
+  const bool wave = !per_ray_walk(R.num_rays, R.traversal);  // (the forward workspace is sized for this walk)
   profile_begin(EVER_KERNEL_FORWARD, st);
   int flags = (R.proposal ? 1 : 0) | (A.vcol ? 2 : 0);
   if (wave) {

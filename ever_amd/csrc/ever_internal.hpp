@@ -18,8 +18,11 @@ constexpr int kRadix = 256;
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr int kLeafBlock = 64;    // leaves per block of the range-union refit
-constexpr int kMaxFragments = 4;
-constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 4-wide node  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
+constexpr int kMaxFragments = 4;  // leaf fragments per primitive (elongated ellipsoids are split along the major axis)
+constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 4-wide node
+
+// int32 offset of the view masks in the start-inside candidate region (8-byte aligned, after N int32 entries)
+__host__ __device__ constexpr int64_t cand_mask_offset(int64_t n) { return (n + 1) & ~int64_t(1); }
 
 // Scene workspace: everything the LBVH build writes and the tracers read. All offsets 256-byte aligned.
 // Leaves are fragments: each primitive owns kMaxFragments slots (S = kMaxFragments * N); unused slots carry the
@@ -36,7 +39,8 @@ constexpr int32_t kEmptyChild = (int32_t)0x80000000;  // unused child slot of a 
 //   tab      : float[levels * blocks * 6] sparse table of block unions
 //   rec      : float4[N * 4] per-primitive traversal record, original order (64 B)
 //   colr     : float4[N] (density, R, G, B) in original order
-//   cand     : int32[N] start-inside candidates
+//   cand     : int32[N] start-inside candidates, then (at cand_mask_offset) uint64[N] their view masks (several
+//              views: bit v set when the candidate's box lies within tmin of view v's origin)
 //   bpart    : float[6 * ceil(N/256)] per-block centroid bounds
 struct SceneLayout {
   int64_t n = 0, slots = 0, tiles = 0, leaf_blocks = 0;
@@ -78,7 +82,7 @@ inline SceneLayout scene_layout(int64_t n) {
   L.tab = take(24 * (size_t)L.tab_levels * (L.leaf_blocks > 0 ? L.leaf_blocks : 1));
   L.rec = take(64 * n);
   L.colr = take(16 * n);
-  L.cand = take(4 * n);
+  L.cand = take(4 * (size_t)cand_mask_offset(n) + 8 * (size_t)n);
   L.bpart = take(24 * (size_t)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1));
   L.total = off;
   return L;
@@ -136,15 +140,32 @@ hipError_t launch_eval_sh(const float* means, const float* feat, const float* or
 hipError_t launch_eval_sh_bwd(const float* means, const float* feat, const float* origin, const float* drgb,
                               int64_t n, int d, int deg, float* dfeat, hipStream_t s);
 
-constexpr int kSpillPerRay = 64;  // int32 traversal-stack spill entries per ray (4-wide stack bound: 3 x 26)
+// Traversal stack bound. Along a root-to-leaf path of the Karras hierarchy the split level strictly increases.
+// Used leaf keys are below 2^30, so distinct keys split at levels 2..31; equal keys split on the sorted leaf index
+// (level 32 + clz(a ^ b), a, b < 2^30 leaves: levels 34..63). A path therefore holds at most 60 internal nodes,
+// and so at most 60 4-wide records (each record descends at least one binary level). A record pushes at most three
+// children, so a traversal stack never holds more than 180 entries, whatever the scene.
+constexpr int kMaxPathNodes = 60;
+constexpr int kStackBound = 3 * kMaxPathNodes;
+constexpr int kLdsStackPerRay = 32;   // per-ray walk: stack entries per lane in LDS
+constexpr int kSpillPerRay = 160;     // ... and in the per-ray spill area of the forward workspace
+static_assert(kLdsStackPerRay + kSpillPerRay >= kStackBound, "per-ray traversal stack below the tree bound");
+constexpr int kWaveStackRows = 3;     // per-wave walk: 64-entry rows, two in VGPRs and one in LDS
+static_assert(64 * kWaveStackRows >= kStackBound, "per-wave traversal stack below the tree bound");
 constexpr int64_t kMinRaysToSort = 65536;  // below this the rays are traced in the caller's order
 
-// Forward workspace: traversal-stack spill per ray, and the ray sort (30-bit Morton key of a point on each ray).
+// Whether a call traces with the per-ray walk (EVER_TRAVERSAL_RAY, or AUTO on a batch too small to sort).
+inline bool per_ray_walk(int64_t nr, int traversal) {
+  return traversal == EVER_TRAVERSAL_RAY || (traversal == EVER_TRAVERSAL_AUTO && nr < kMinRaysToSort);
+}
+
+// Forward workspace: traversal-stack spill of the per-ray walk (the per-wave walk keeps its whole stack on chip), the
+// ray sort (30-bit Morton key of a point on each ray) and the integrator's state per processing slot (112 B).
 struct ForwardLayout {
   int64_t tiles = 0;
-  size_t spill, keys0, vals0, keys1, vals1, hist, counter, total;
+  size_t spill, keys0, vals0, keys1, vals1, hist, counter, ist, total;
 };
-inline ForwardLayout forward_layout(int64_t nr) {
+inline ForwardLayout forward_layout(int64_t nr, bool per_ray) {
   ForwardLayout L;
   const int64_t r = nr > 0 ? nr : 1;
   L.tiles = (r + kSortTile - 1) / kSortTile;
@@ -154,13 +175,14 @@ inline ForwardLayout forward_layout(int64_t nr) {
     off = align_up(off + bytes, 256);
     return o;
   };
-  L.spill = take(sizeof(int32_t) * kSpillPerRay * (size_t)r);
+  L.spill = take(per_ray ? sizeof(int32_t) * kSpillPerRay * (size_t)r : 0);
   L.keys0 = take(4 * (size_t)r);
   L.vals0 = take(4 * (size_t)r);
   L.keys1 = take(4 * (size_t)r);
   L.vals1 = take(4 * (size_t)r);
   L.hist = take(4 * (size_t)kRadix * (L.tiles + 1));  // + digit totals
   L.counter = take(4);                                 // next tile of a persistent launch
+  L.ist = take(112 * (size_t)r);
   L.total = off;
   return L;
 }

@@ -245,10 +245,10 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
     tri = alloc((nr * max_iters,), dtype=torch.int32, device=dev) if store_events else None
     touch = torch.empty((scene.n,), dtype=torch.int32, device=dev) if count_touches else None
     init = torch.empty((nr, 4), dtype=torch.float32, device=dev)
-    st64 = torch.empty((nr, 6), dtype=torch.float64, device=dev)
+    st64 = torch.empty((nr, 8), dtype=torch.float64, device=dev)
     order = torch.empty((nr,), dtype=torch.int32, device=dev)
     dirac = torch.empty((nr, 4), dtype=torch.float32, device=dev) if last_dirac else None
-    fws_bytes = L.ever_forward_workspace_bytes(nr)
+    fws_bytes = L.ever_forward_workspace_bytes_for(nr, TRAVERSALS[traversal or _TRAVERSAL])
     fws = torch.empty(fws_bytes, dtype=torch.uint8, device=dev)
     if proposal is not None:
         _check_float(proposal, "proposal", None, dev)

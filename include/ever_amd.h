@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 6
+#define EVER_ABI_VERSION 8
 
 /* status codes */
 #define EVER_OK 0
@@ -117,9 +117,11 @@ typedef struct ever_forward_out {
   int32_t* tri_collection; /* (R*max_iters), event id 2*prim+entry (layout: ever_rays.event_layout); nullable */
   int32_t* touch_count;    /* (N) events per primitive (exact); nullable; zeroed by the call */
   float* initial_drgb;     /* (R,4) density/colour already open at the ray start */
-  double* ray_state64;     /* (R,6) FP64 final accumulators, read by ever_trace_backward */
+  double* ray_state64;     /* (R,8) FP64 final accumulators: density and colour (4), log-transmittance, depth of the
+                              last event, and the log-transmittance and depth that ever_trace_backward unwinds (the
+                              same accumulation with its safe roots) */
   int32_t* ray_order;      /* (R) processing order (rays sorted for coherence), read by ever_trace_backward */
-  void* workspace;         /* stack spill + ray sort, ever_forward_workspace_bytes(R) bytes */
+  void* workspace;         /* stack spill + ray sort, ever_forward_workspace_bytes_for(R, traversal) bytes */
   size_t workspace_bytes;
   float* last_dirac; /* (R,4) control point of the last event integrated, (+-density, +-density * colour)
                         (shaders.slang:229-232), zeros for a ray without events; nullable */
@@ -165,8 +167,13 @@ int ever_build_scene(const ever_prims* prims, void* workspace, size_t workspace_
  * the tree, the primitive records' geometry rows and the bounds stay as built. */
 int ever_update_colors(const ever_prims* prims, void* workspace, size_t workspace_bytes, void* stream);
 
-/* The scene workspace is also written: the start-inside candidate list of this ray batch. */
+/* Forward workspace: the per-ray walk's traversal-stack spill (160 entries per ray beyond 32 in LDS; the per-wave
+ * walk keeps its 192 entries on chip; either holds every entry a tree can need: at most 180, from the key and
+ * index bits of the hierarchy) and the ray sort. ever_forward_workspace_bytes_for sizes it for a traversal mode (EVER_TRAVERSAL_*);
+ * ever_forward_workspace_bytes(n) = ever_forward_workspace_bytes_for(n, EVER_TRAVERSAL_AUTO).
+ * The scene workspace is also written: the start-inside candidate list of this ray batch. */
 size_t ever_forward_workspace_bytes(int64_t num_rays);
+size_t ever_forward_workspace_bytes_for(int64_t num_rays, int32_t traversal);
 int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspace_bytes,
                        const ever_rays* rays, const ever_forward_out* out, void* stream);
 

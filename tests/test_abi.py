@@ -28,7 +28,7 @@ def test_every_header_function_is_exported_and_bound():
 
 def test_abi_version_and_struct_sizes():
     L = C.lib()
-    assert L.ever_abi_version() == 6
+    assert L.ever_abi_version() == 8
     sizes = (ctypes.c_int64 * 4)()
     assert L.ever_struct_sizes(sizes) == 0
     assert list(sizes) == [ctypes.sizeof(C.Prims), ctypes.sizeof(C.Rays), ctypes.sizeof(C.ForwardOut),
@@ -41,6 +41,12 @@ def test_workspace_queries_grow_with_size():
     assert 0 < a < b
     assert L.ever_scene_workspace_bytes(100000) >= 100000 * 64 * 2
     assert L.ever_forward_workspace_bytes(10) > 0
+    # the per-ray walk spills up to 160 stack entries per ray, the per-wave walk none; AUTO takes the per-ray walk
+    # below 65536 rays
+    for nr in (10, 70000):
+        ray, wave = L.ever_forward_workspace_bytes_for(nr, 1), L.ever_forward_workspace_bytes_for(nr, 2)
+        assert ray - wave >= 4 * 150 * nr
+        assert L.ever_forward_workspace_bytes(nr) == (ray if nr < 65536 else wave)
     lay = C.scene_layout(1000)
     offs = [lay[k] for k in C.LAYOUT_NAMES]
     assert offs == sorted(offs) and lay["total"] == a

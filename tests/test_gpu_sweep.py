@@ -1,11 +1,13 @@
 """Randomised parity sweep of the HIP tracer against the FP64 oracle: seeded draws of scene kind and size,
 feature degree, camera (outside or inside the scene), tmin, tmax, max_iters, k-buffer size and tree walk, each
-checked forward (colour 1e-4, opacity, distortion, per-ray event counts) and backward (every gradient tensor: 1e-3
-relative), each bar raised to twice the error of the same algorithm evaluated in float32 where a case is worse
-conditioned than that.
+checked forward (colour and opacity 1e-4 absolute, distortion 1e-4 relative, per-ray event counts exact) and
+backward (every gradient tensor: 1e-3 relative L2 error; the largest single error 1e-3 of the tensor's largest
+element, raised up to 1e-2 where the same algorithm evaluated in float32 is itself off by more than half that,
+i.e. scale gradients along the thin axis of needle-like primitives).
 Cases with a tmax that cuts through primitives and k-buffer sizes other than 16 check that results follow the
 reference's 16-event rounds whatever the buffer size."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -62,7 +64,21 @@ def _draw(seed):
                      f"tmax {tmax:.2f} max_iters {max_iters} K {kbuf} {traversal} wcts {wcts is not None}")
 
 
-@pytest.mark.parametrize("seed", range(96))
+# seeds beyond the first 96 that once failed: grazing hits, near-tied events ordered differently by float32 and FP64
+# depths, exits beyond a cutting tmax at the reference's round boundaries, thin-axis scale gradients
+HARD_SEEDS = (138, 156, 180, 204, 289, 311, 330, 342, 346, 348, 354, 369, 438, 468, 494, 496, 513, 522, 546)
+
+
+def _seeds():
+    # the committed seeds; EVER_SWEEP_SEEDS="a-b" draws seeds a..b instead (wider searches)
+    spec = os.environ.get("EVER_SWEEP_SEEDS")
+    if not spec:
+        return list(range(96)) + list(HARD_SEEDS)
+    a, b = (int(x) for x in spec.split("-"))
+    return range(a, b + 1)
+
+
+@pytest.mark.parametrize("seed", _seeds())
 def test_random_configuration_matches_oracle(seed):
     c = _draw(seed)
     print("case:", c["desc"], flush=True)
@@ -100,7 +116,7 @@ def test_random_configuration_matches_oracle(seed):
     mism = float((iters != ref["iters"]).mean())
     print(f"colour max abs {err:.2e} (float32 restatement {f_err:.2e}), distortion rel {derr:.2e} ({f_derr:.2e}), "
           f"iters mismatch {mism:.4f}, events/ray {iters.mean():.1f}", flush=True)
-    assert err <= max(1e-4, 2.0 * f_err) and derr <= max(1e-4, 2.0 * f_derr) and mism <= 0.005, (err, f_err, derr)
+    assert err <= 1e-4 and derr <= 1e-4 and mism == 0.0, (err, derr, mism)
     w64 = None if c["wcts"] is None else c["wcts"].astype(np.float64)
     rb = oracle.backward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["max_iters"],
                          c["g"].double().numpy(), ref, wcts=w64)
@@ -116,6 +132,4 @@ def test_random_configuration_matches_oracle(seed):
         f_mx = np.abs(r32k - r).max() / max(np.abs(r).max(), 1e-12)
         print(f"  {k}: L2 rel {e_l2:.2e}, max rel {e_mx:.2e} (float32 restatement: {f_l2:.2e}, {f_mx:.2e})",
               flush=True)
-        # bar: 1e-3 relative, or twice the float32 restatement's own error where the case is worse conditioned
-        # (quaternion gradients of grazing hits; scale gradients of thin, dense primitives)
-        assert e_l2 <= max(1e-3, 2.0 * f_l2) and e_mx <= max(1e-3, 2.0 * f_mx), k
+        assert e_l2 <= 1e-3 and e_mx <= min(max(1e-3, 2.0 * f_mx), 1e-2), k
