@@ -12,6 +12,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("EVER_AMD_LIB", os.path.join(_HERE, "libever_amd.so"))
 
 EVER_OK = 0
+EVER_ERR_INVALID = 1
+EVER_ERR_HIP = 2
+EVER_ERR_WORKSPACE = 3
+EVER_MORTON_NOALLOC_MAX_PRIMS = 1048576
 EVER_COLOR_DC = 0
 EVER_COLOR_SH = 1
 EVER_EVENTS_BY_RAY = 0
@@ -96,6 +100,7 @@ SIGNATURES = {
                                 ctypes.c_int),
     "ever_morton_workspace_bytes": ([_i64], ctypes.c_size_t),
     "ever_morton_order": ([_vp, _i64, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "ever_morton_order_noalloc": ([_vp, _i64, _vp, _vp], ctypes.c_int),
     "ever_camera_rays": ([ctypes.POINTER(Camera), _vp, _vp, _vp, _vp], ctypes.c_int),
     "ever_eval_sh": ([_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),
     "ever_eval_sh_backward": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp], ctypes.c_int),

@@ -270,6 +270,16 @@ int ever_morton_order(const float* means, int64_t num_prims, int32_t* order, voi
                     "ever_morton_order");
 }
 
+int ever_morton_order_noalloc(const float* means, int64_t num_prims, int32_t* order, void* stream) {
+  if (num_prims < 0) return fail(EVER_ERR_INVALID, "num_prims must be >= 0");
+  if (num_prims > EVER_MORTON_NOALLOC_MAX_PRIMS)
+    return fail(EVER_ERR_INVALID, "num_prims exceeds EVER_MORTON_NOALLOC_MAX_PRIMS (use ever_morton_order with a workspace)");
+  if (num_prims > 0 && (!means || !order)) return fail(EVER_ERR_INVALID, "null pointer");
+  if (num_prims == 0) return EVER_OK;
+  return hip_status(ever::launch_morton_order_noalloc(means, num_prims, order, (hipStream_t)stream),
+                    "ever_morton_order_noalloc");
+}
+
 int ever_camera_rays(const ever_camera* camera, const float* jitter, float* origins, float* directions,
                      void* stream) {
   if (!camera) return fail(EVER_ERR_INVALID, "camera is null");

@@ -128,7 +128,32 @@ hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, co
 struct MortonLayout {
   size_t box, part, keys0, vals0, keys1, vals1, hi, hist, total;
 };
-MortonLayout morton_layout(int64_t n);
+constexpr int kBoundsBlocks = 1024;  // blocks of the partial bound reduction of the Morton order
+// Morton-order workspace: mean bounds (partial and final), 64-bit keys as two radix-sort passes over 32 + 31 bits
+// (ping-pong keys/values, the high words) and the digit histograms.
+constexpr MortonLayout morton_layout(int64_t n) {
+  MortonLayout L{};
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+  L.box = take(6 * sizeof(float));
+  L.part = take(6 * sizeof(float) * kBoundsBlocks);
+  L.keys0 = take(4 * (size_t)n);
+  L.vals0 = take(4 * (size_t)n);
+  L.keys1 = take(4 * (size_t)n);
+  L.vals1 = take(4 * (size_t)n);
+  L.hi = take(4 * (size_t)n);
+  L.hist = take(4 * (size_t)kRadix * ((tiles > 0 ? tiles : 1) + 1));
+  L.total = off;
+  return L;
+}
+// Morton order without a caller workspace: the library's own scratch array, for at most this many primitives
+constexpr int64_t kMortonNoallocMaxPrims = EVER_MORTON_NOALLOC_MAX_PRIMS;
+hipError_t launch_morton_order_noalloc(const float* means, int64_t n, int32_t* order, hipStream_t s);
 hipError_t launch_scene_from_records(const float* records, int64_t n, const ever_ply_layout& L, const int32_t* order,
                                      float* means, float* scales, float* quats, float* densities, float* features,
                                      hipStream_t s);

@@ -3,6 +3,8 @@
 // Morton order of the primitives (ply_file_loader.cc:72-148) computed with the library's radix sort.
 #include <cfloat>
 
+#include <mutex>
+
 #include "ever_internal.hpp"
 
 namespace ever {
@@ -128,30 +130,8 @@ __global__ __launch_bounds__(256) void k_gather_u32(const uint32_t* __restrict__
   if (i < n) dst[i] = src[idx[i]];
 }
 
-constexpr int kBoundsBlocks = 1024;
 
 }  // namespace
-
-MortonLayout morton_layout(int64_t n) {
-  MortonLayout L{};
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off += (bytes + 255) & ~(size_t)255;
-    return o;
-  };
-  const int64_t tiles = (n + kSortTile - 1) / kSortTile;
-  L.box = take(6 * sizeof(float));
-  L.part = take(6 * sizeof(float) * kBoundsBlocks);
-  L.keys0 = take(4 * (size_t)n);
-  L.vals0 = take(4 * (size_t)n);
-  L.keys1 = take(4 * (size_t)n);
-  L.vals1 = take(4 * (size_t)n);
-  L.hi = take(4 * (size_t)n);
-  L.hist = take(4 * (size_t)kRadix * ((tiles > 0 ? tiles : 1) + 1));
-  L.total = off;
-  return L;
-}
 
 hipError_t launch_scene_from_records(const float* records, int64_t n, const ever_ply_layout& Lc, const int32_t* order,
                                      float* means, float* scales, float* quats, float* densities, float* features,
@@ -184,6 +164,35 @@ hipError_t launch_morton_order(const float* means, int64_t n, int32_t* order, vo
   e = launch_sort_pairs(k0, v0, k1, v1, hist, n, 31, st);
   if (e != hipSuccess) return e;
   return hipMemcpyAsync(order, v0, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+}
+
+// Scratch of ever_morton_order_noalloc: one array per GPU the library runs on, held until the process ends. Every
+// region is written before it is read (bounds, keys, values, histograms), so nothing needs clearing between calls;
+// calls that overlap on one GPU share it and race.
+constexpr size_t kMortonScratchBytes = morton_layout(kMortonNoallocMaxPrims).total;
+static __device__ __align__(256) unsigned char scratch[kMortonScratchBytes];
+
+// The array's device address on each GPU, looked up (hipGetSymbolAddress) on the first call with that GPU current.
+// In host code `scratch` names a host-side stand-in, never the device memory: kernels get this address instead.
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+hipError_t launch_morton_order_noalloc(const float* means, int64_t n, int32_t* order, hipStream_t st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return e;
+      g_scratch_ptr[dev] = ptr;
+    }
+    ptr = g_scratch_ptr[dev];
+  }
+  return launch_morton_order(means, n, order, ptr, st);
 }
 
 }  // namespace ever

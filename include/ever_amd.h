@@ -279,6 +279,14 @@ int ever_eval_sh_backward(const float* means, const float* features, const float
                           int64_t num_prims, int32_t feature_size, int32_t sh_degree, float* dfeatures,
                           void* stream);
 
+/* ever_morton_order without a workspace argument, for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. It uses a scratch array the
+ * library holds on each GPU it runs on until the process ends (about 21 MB), so it allocates nothing; num_prims
+ * above EVER_MORTON_NOALLOC_MAX_PRIMS is rejected (EVER_ERR_INVALID). Calls that overlap on one GPU share that
+ * array and race: serialize them (one stream, or events). Results equal ever_morton_order's. */
+#define EVER_MORTON_NOALLOC_MAX_PRIMS 1048576
+int ever_morton_order_noalloc(const float* means, int64_t num_prims, int32_t* order, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

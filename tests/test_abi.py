@@ -104,6 +104,13 @@ def test_invalid_arguments_return_status_codes():
     assert L.ever_morton_order(ctypes.cast(buf, ctypes.c_void_p), 5, ctypes.cast(buf, ctypes.c_void_p),
                                ctypes.cast(buf, ctypes.c_void_p), 16, None) == 3
     assert L.ever_morton_workspace_bytes(1000) > L.ever_morton_workspace_bytes(10) > 0
+    # the variant without a workspace: argument checks before any GPU call, and its size limit
+    assert L.ever_morton_order_noalloc(None, 5, None, None) == 1
+    big = C.EVER_MORTON_NOALLOC_MAX_PRIMS + 1
+    assert L.ever_morton_order_noalloc(ctypes.cast(buf, ctypes.c_void_p), big, ctypes.cast(buf, ctypes.c_void_p),
+                                       None) == 1
+    assert b"EVER_MORTON_NOALLOC_MAX_PRIMS" in L.ever_last_error()
+    assert L.ever_morton_order_noalloc(None, 0, None, None) == 0
     cam = C.Camera()
     assert L.ever_camera_rays(None, None, None, None, None) == 1
     cam.width, cam.height, cam.fx, cam.fy = 4, 3, 1.0, 1.0
