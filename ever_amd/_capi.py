@@ -96,6 +96,9 @@ SIGNATURES = {
     "ever_backward_workspace_bytes": ([_i64, _i64, _i64], ctypes.c_size_t),
     "ever_trace_backward": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
                              ctypes.POINTER(BackwardIO), _vp], ctypes.c_int),
+    "ever_backward_recompute_workspace_bytes": ([_i64, _i64, _i64, _i32, _i32, _i64], ctypes.c_size_t),
+    "ever_trace_backward_recompute": ([ctypes.POINTER(Prims), _vp, ctypes.c_size_t, ctypes.POINTER(Rays),
+                                       ctypes.POINTER(BackwardIO), _i64, _vp], ctypes.c_int),
     "ever_scene_from_records": ([_vp, _i64, ctypes.POINTER(PlyLayout), _vp, _vp, _vp, _vp, _vp, _vp, _vp],
                                 ctypes.c_int),
     "ever_morton_workspace_bytes": ([_i64], ctypes.c_size_t),

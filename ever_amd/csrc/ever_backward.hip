@@ -44,6 +44,8 @@ struct BwdArgs {
   float* dinit;      // (R,4) scratch or caller buffer
   const int32_t* order;  // processing order (nullable: identity)
   int32_t events_by_slot;  // tri column = processing slot instead of ray index
+  int64_t ray0;            // the slots cover rays ray0 .. ray0 + nr - 1 (order entries are relative to ray0)
+  int64_t tri_stride;      // row stride of the event list (one row per event index)
   // several views: per-view (density, colour) records, each ray's view, per-view colour gradients (V,N,4)
   const float4* vcol;
   const int32_t* view_index;
@@ -145,15 +147,16 @@ __device__ __forceinline__ void event_load(const BwdArgs& A, int32_t id, int32_t
   R.scl = ld3(A.scales + 3 * p);
 }
 
-// One event's primitive and safe roots (FP64 depth, FP32 intermediates for the adjoint). The forward accumulated
-// its log-transmittance for this kernel with the same depths (safe_roots_d), so the states unwind exactly. The loop
-// below needs them twice, as the depth of the previous event and then as the event whose adjoint is taken; they
-// are computed once and carried over.
+// One event's primitive and its selected safe root: the FP64 depth, and the ray in the primitive's normalized
+// frame that the adjoint of the root starts from. The forward accumulated its log-transmittance for this kernel
+// with the same depths (safe_roots_d), so the states unwind exactly. The loop below needs them twice, as the depth
+// of the previous event and then as the event whose adjoint is taken; they are computed once and carried over.
 struct EventGeo {
-  SafeRoots S;
+  d3 ocn, rdn;  // rotated origin and unit direction divided by the scales (FP64)
   float4 q, cr;
   f3 mean, scl, ro;  // ro = (float)(origin - mean)
   bool ok;           // the ray meets the ellipsoid
+  bool sel_r0;       // the selected root is r0 = c / q
 };
 __device__ __forceinline__ double event_geo(d3 org, f3 dir, int32_t id, const EventRaw& R, EventGeo& G) {
   G.q = R.q;
@@ -162,8 +165,15 @@ __device__ __forceinline__ double event_geo(d3 org, f3 dir, int32_t id, const Ev
   G.scl = R.scl;
   const d3 rod = org - tod(G.mean);
   G.ro = tof(rod);
-  G.ok = safe_roots(rod, dir, G.scl, G.q, G.S);
-  return safe_root_select(G.S, id & 1);
+  SafeRootsD D;
+  safe_roots_d(rod, dir, G.scl, G.q, D);
+  G.ocn = D.ocn;
+  G.rdn = D.rdn;
+  G.ok = D.hit;
+  // entry (face 1): the smaller root, exit: the larger (safe_intersect, tri-intersect.slang:443-456)
+  G.sel_r0 = (id & 1) ? (D.r0 < D.r1) : (D.r0 > D.r1);
+  if (!D.hit) return -1.0;
+  return (id & 1) ? fmin(D.r0, D.r1) : fmax(D.r0, D.r1);
 }
 
 __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_t col, uint32_t* hkeys,
@@ -208,9 +218,9 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   // software pipeline over the events k = E-1 .. 0: event ids are read two events ahead, primitive data one
   // event ahead, so the loads of iteration k are issued during iteration k + 1
   const int32_t* tcol = A.tri + col;
-  int32_t id = tcol[(int64_t)(E - 1) * A.nr];
-  int32_t id_next = E >= 2 ? tcol[(int64_t)(E - 2) * A.nr] : 0;   // event k - 1
-  int32_t id_next2 = E >= 3 ? tcol[(int64_t)(E - 3) * A.nr] : 0;  // event k - 2
+  int32_t id = tcol[(int64_t)(E - 1) * A.tri_stride];
+  int32_t id_next = E >= 2 ? tcol[(int64_t)(E - 2) * A.tri_stride] : 0;   // event k - 1
+  int32_t id_next2 = E >= 3 ? tcol[(int64_t)(E - 3) * A.tri_stride] : 0;  // event k - 2
   // contributions of consecutive events of one primitive (its exit and entry are often neighbours) are summed
   // in registers and reach the block's table once: its float atomics in LDS are the adjoint's dominant cost
   uint32_t pend_key = kEmptySlot;
@@ -230,7 +240,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     EventRaw Rn2;  // raw data of event k - 2 and the id of event k - 3, for the next iterations
     int32_t id_next3 = 0;
     if (k >= 2) event_load(A, id_next2, view, Rn2);
-    if (k >= 3) id_next3 = tcol[(int64_t)(k - 3) * A.nr];
+    if (k >= 3) id_next3 = tcol[(int64_t)(k - 3) * A.tri_stride];
     int32_t id_prev = 0;
     double tprev = 0.0;
     EventGeo Gp;
@@ -303,7 +313,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     const f3 mean = G.mean;
     f3 rob = mk3(0.f, 0.f, 0.f), rdb = mk3(0.f, 0.f, 0.f), sbar = mk3(0.f, 0.f, 0.f);
     float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (G.ok) safe_roots_bwd(G.S, G.ro, dir, G.scl, G.q, face, tb, rob, rdb, sbar, qb);
+    if (G.ok) safe_roots_bwd_d(G.ocn, G.rdn, G.sel_r0, G.ro, dir, G.scl, G.q, tb, rob, rdb, sbar, qb);
     orgb = orgb + rob;
     dirb = dirb + rdb;
     {
@@ -367,8 +377,8 @@ __global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
   __syncthreads();
   const int64_t slot = (int64_t)blockIdx.x * kBwdBlock + threadIdx.x;
   if (slot < A.nr) {
-    const int64_t r = A.order ? (int64_t)A.order[slot] : slot;
-    backward_ray(A, r, A.events_by_slot ? slot : r, hkeys, hvals);
+    const int64_t loc = A.order ? (int64_t)A.order[slot] : slot;
+    backward_ray(A, A.ray0 + loc, A.events_by_slot ? slot : loc, hkeys, hvals);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
@@ -604,96 +614,187 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs E) {
   }
 }
 
+namespace {
+
+// Regions and arguments shared by the stored-event and the recomputing adjoint.
+struct BackwardCall {
+  int64_t n, nr, nw;
+  bool views;
+  float* acc;
+  float* inv;
+  float* dinit;
+};
+
+BackwardCall backward_begin(const ever_prims& P, const ever_rays& R, const ever_backward_io& B, hipStream_t st) {
+  BackwardCall C;
+  C.n = P.num_prims;
+  C.nr = R.num_rays;
+  C.nw = B.wcts ? B.num_wcts : 0;
+  BackwardLayout BL = backward_layout(C.n, C.nr, C.nw);
+  C.acc = at<float>(B.workspace, BL.acc);
+  C.inv = at<float>(B.workspace, BL.inv);
+  C.dinit = B.dinitial_drgb ? B.dinitial_drgb : at<float>(B.workspace, BL.dinit);
+  C.views = P.num_views > 1 && C.n > 0;
+  if (C.n > 0) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(16 * C.n, 256)), dim3(256), 0, st, C.acc, 16 * C.n);
+  if (C.views)
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(4 * C.n * P.num_views, 256)), dim3(256), 0, st, B.view_color_grads,
+                       4 * C.n * (int64_t)P.num_views);
+  if (C.nw > 0) hipLaunchKernelGGL(k_invert4, dim3(grid_for(C.nw, 64)), dim3(64), 0, st, B.wcts, C.nw, C.inv);
+  return C;
+}
+
+// Arguments of k_trace_backward over all rays of the call, events read from the caller's stored list.
+BwdArgs backward_args(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
+                      const BackwardCall& C) {
+  SceneLayout L = scene_layout(C.n);
+  BwdArgs A;
+  A.means = P.means;
+  A.scales = P.scales;
+  A.quats = P.quats;
+  A.colr = at<float4>(ws, L.colr);
+  A.rayo = R.origins;
+  A.rayd = R.directions;
+  A.nr = C.nr;
+  A.tmin = R.tmin;
+  A.max_iters = R.max_iters;
+  A.grad_out = B.grad_out;
+  A.iters = B.iters;
+  A.tri = B.tri_collection;
+  A.states = B.states;
+  A.st64 = B.ray_state64;
+  A.wcts = B.wcts;
+  A.nw = C.nw;
+  A.inv = C.inv;
+  A.acc = C.acc;
+  A.drayo = B.drayo;
+  A.drayd = B.drayd;
+  A.dinit = C.dinit;
+  A.order = B.ray_order;
+  A.events_by_slot = R.event_layout == EVER_EVENTS_BY_SLOT ? 1 : 0;
+  A.ray0 = 0;
+  A.tri_stride = C.nr;
+  A.vcol = C.views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
+  A.view_index = C.views ? R.view_index : nullptr;
+  A.vgrad = C.views ? B.view_color_grads : nullptr;
+  A.n = C.n;
+  return A;
+}
+
+void backward_trace(const BwdArgs& A, hipStream_t st) {
+  profile_begin(EVER_KERNEL_BACKWARD, st);
+  hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(A.nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
+  profile_end(EVER_KERNEL_BACKWARD, st);
+}
+
+// Start-inside adjoint over all rays, then the per-primitive epilogue (clamps, means2D, SH feature gradients).
+void backward_end(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
+                  const BackwardCall& C, hipStream_t st) {
+  const int64_t n = C.n;
+  SceneLayout L = scene_layout(n);
+  BackwardLayout BL = backward_layout(n, C.nr, C.nw);
+  const float4* vcol = C.views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
+  if (C.nr > 0 && n > 0)
+    hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(C.nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
+                       at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,
+                       R.directions, C.nr, R.tmin, C.dinit, C.acc, vcol, C.views ? R.view_index : nullptr,
+                       P.view_origins, C.views ? B.view_color_grads : nullptr, n);
+  if (n <= 0) return;
+  EpiArgs E;
+  E.acc = C.acc;
+  E.means = P.means;
+  E.colr = at<float4>(ws, L.colr);
+  E.n = n;
+  E.d = P.feature_size;
+  E.color_mode = P.color_mode;
+  E.sh_degree = P.sh_degree;
+  E.activation = P.activation;
+  E.ox = P.sh_origin[0];
+  E.oy = P.sh_origin[1];
+  E.oz = P.sh_origin[2];
+  E.wcts = B.wcts;
+  E.inv = C.inv;
+  E.nw = C.nw;
+  E.dmeans = B.dmeans;
+  E.dscales = B.dscales;
+  E.dquats = B.dquats;
+  E.ddens = B.ddensities;
+  E.dfeat = B.dfeatures;
+  E.dmeans2D = C.nw > 0 ? B.dmeans2D : nullptr;
+  E.grgb = at<float>(B.workspace, BL.grgb);
+  E.vorig = C.views ? P.view_origins : nullptr;
+  E.vcol = vcol;
+  E.nv = C.views ? P.num_views : 0;
+  E.vgrad = C.views ? B.view_color_grads : nullptr;
+  hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);
+  if (P.color_mode == EVER_COLOR_SH) {
+    if (C.views)
+      hipLaunchKernelGGL(k_sh_feature_grads_views, dim3(grid_for(n, 256)), dim3(256), 0, st, B.view_color_grads,
+                         P.means, P.view_origins, P.num_views, n, P.feature_size, P.sh_degree, B.dfeatures);
+    else
+      hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n, 256)), dim3(256), 0, st, E.grgb, P.means,
+                         P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size, P.sh_degree,
+                         B.dfeatures);
+  }
+}
+
+}  // namespace
+
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
                            hipStream_t st) {
-  const int64_t n = P.num_prims, nr = R.num_rays;
-  SceneLayout L = scene_layout(n);
-  BackwardLayout BL = backward_layout(n, nr, B.wcts ? B.num_wcts : 0);
-  float* acc = at<float>(B.workspace, BL.acc);
-  float* inv = at<float>(B.workspace, BL.inv);
-  float* dinit = B.dinitial_drgb ? B.dinitial_drgb : at<float>(B.workspace, BL.dinit);
-  const int64_t nw = B.wcts ? B.num_wcts : 0;
-  if (n > 0) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(16 * n, 256)), dim3(256), 0, st, acc, 16 * n);
-  const bool views = P.num_views > 1 && n > 0;
-  if (views)
-    hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(4 * n * P.num_views, 256)), dim3(256), 0, st, B.view_color_grads,
-                       4 * n * (int64_t)P.num_views);
-  if (nw > 0) hipLaunchKernelGGL(k_invert4, dim3(grid_for(nw, 64)), dim3(64), 0, st, B.wcts, nw, inv);
-  if (nr > 0) {
-    BwdArgs A;
-    A.means = P.means;
-    A.scales = P.scales;
-    A.quats = P.quats;
-    A.colr = at<float4>(ws, L.colr);
-    A.rayo = R.origins;
-    A.rayd = R.directions;
-    A.nr = nr;
-    A.tmin = R.tmin;
-    A.max_iters = R.max_iters;
-    A.grad_out = B.grad_out;
-    A.iters = B.iters;
-    A.tri = B.tri_collection;
-    A.states = B.states;
-    A.st64 = B.ray_state64;
-    A.wcts = B.wcts;
-    A.nw = nw;
-    A.inv = inv;
-    A.acc = acc;
-    A.drayo = B.drayo;
-    A.drayd = B.drayd;
-    A.dinit = dinit;
-    A.order = B.ray_order;
-    A.events_by_slot = R.event_layout == EVER_EVENTS_BY_SLOT ? 1 : 0;
-    A.vcol = views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
-    A.view_index = views ? R.view_index : nullptr;
-    A.vgrad = views ? B.view_color_grads : nullptr;
-    A.n = n;
-    profile_begin(EVER_KERNEL_BACKWARD, st);
-    hipLaunchKernelGGL(k_trace_backward, dim3(grid_for(nr, kBwdBlock)), dim3(kBwdBlock), 0, st, A);
-    profile_end(EVER_KERNEL_BACKWARD, st);
-    if (n > 0)
-      hipLaunchKernelGGL(k_init_adjoint, dim3(grid_for(nr, 256)), dim3(256), 0, st, P.means, P.scales, P.quats,
-                         at<float4>(ws, L.colr), at<int32_t>(ws, L.cand), at<uint32_t>(ws, L.header), R.origins,
-                         R.directions, nr, R.tmin, dinit, acc, A.vcol, A.view_index, P.view_origins, A.vgrad, n);
-  }
-  if (n > 0) {
-    EpiArgs E;
-    E.acc = acc;
-    E.means = P.means;
-    E.colr = at<float4>(ws, L.colr);
-    E.n = n;
-    E.d = P.feature_size;
-    E.color_mode = P.color_mode;
-    E.sh_degree = P.sh_degree;
-    E.activation = P.activation;
-    E.ox = P.sh_origin[0];
-    E.oy = P.sh_origin[1];
-    E.oz = P.sh_origin[2];
-    E.wcts = B.wcts;
-    E.inv = inv;
-    E.nw = nw;
-    E.dmeans = B.dmeans;
-    E.dscales = B.dscales;
-    E.dquats = B.dquats;
-    E.ddens = B.ddensities;
-    E.dfeat = B.dfeatures;
-    E.dmeans2D = nw > 0 ? B.dmeans2D : nullptr;
-    E.grgb = at<float>(B.workspace, BL.grgb);
-    E.vorig = views ? P.view_origins : nullptr;
-    E.vcol = views ? reinterpret_cast<const float4*>(P.view_colors) : nullptr;
-    E.nv = views ? P.num_views : 0;
-    E.vgrad = views ? B.view_color_grads : nullptr;
-    hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, st, E);
-    if (P.color_mode == EVER_COLOR_SH) {
-      if (views)
-        hipLaunchKernelGGL(k_sh_feature_grads_views, dim3(grid_for(n, 256)), dim3(256), 0, st, B.view_color_grads,
-                           P.means, P.view_origins, P.num_views, n, P.feature_size, P.sh_degree, B.dfeatures);
-      else
-        hipLaunchKernelGGL(k_sh_feature_grads, dim3(grid_for(n, 256)), dim3(256), 0,
-                           st, E.grgb, P.means, P.sh_origin[0], P.sh_origin[1], P.sh_origin[2], n, P.feature_size,
-                           P.sh_degree, B.dfeatures);
+  const BackwardCall C = backward_begin(P, R, B, st);
+  if (C.nr > 0) backward_trace(backward_args(P, ws, R, B, C), st);
+  backward_end(P, ws, R, B, C, st);
+  return hipGetLastError();
+}
+
+// The adjoint with its event lists traced again, one chunk of rays at a time: the chunk's forward (with the
+// start-inside candidates of the call's own forward, still in the scene workspace) writes the chunk's events by
+// processing slot into a list of chunk x max_iters entries, and k_trace_backward consumes it before the next chunk
+// overwrites it. A ray's events do not depend on the other rays traced with it, so every ray sees the events of
+// the stored-list adjoint; the accumulators, the start-inside adjoint and the epilogue are shared by all chunks.
+hipError_t launch_backward_recompute(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
+                                     int64_t chunk, hipStream_t st) {
+  const BackwardCall C = backward_begin(P, R, B, st);
+  if (C.nr > 0) {
+    const RecomputeLayout RL = recompute_layout(C.n, C.nr, C.nw, R.max_iters, R.traversal, chunk);
+    BwdArgs A = backward_args(P, ws, R, B, C);
+    int32_t* tri = at<int32_t>(B.workspace, RL.tri);
+    A.tri = tri;
+    A.events_by_slot = 1;
+    for (int64_t a = 0; a < C.nr; a += RL.chunk) {
+      const int64_t c = std::min<int64_t>(RL.chunk, C.nr - a);
+      ever_rays Rc = R;
+      Rc.origins = R.origins + 3 * a;
+      Rc.directions = R.directions + 3 * a;
+      Rc.num_rays = c;
+      Rc.proposal = R.proposal ? R.proposal + a : nullptr;
+      Rc.view_index = R.view_index ? R.view_index + a : nullptr;
+      Rc.tile = 0;
+      Rc.event_layout = EVER_EVENTS_BY_SLOT;
+      // the chunk's forward workspace is sized for the walk of a full chunk; a shorter last chunk keeps that walk
+      Rc.traversal = per_ray_walk(RL.chunk, R.traversal) ? EVER_TRAVERSAL_RAY : EVER_TRAVERSAL_WAVE;
+      ever_forward_out O{};
+      O.color = at<float>(B.workspace, RL.color);
+      O.states = at<float>(B.workspace, RL.states);
+      O.iters = at<int32_t>(B.workspace, RL.iters);
+      O.tri_collection = tri;
+      O.touch_count = nullptr;
+      O.initial_drgb = at<float>(B.workspace, RL.init);
+      O.ray_state64 = at<double>(B.workspace, RL.st64);
+      O.ray_order = at<int32_t>(B.workspace, RL.order);
+      O.workspace = at<char>(B.workspace, RL.fwd);
+      O.workspace_bytes = forward_layout(RL.chunk, per_ray_walk(RL.chunk, R.traversal)).total;
+      O.last_dirac = nullptr;
+      hipError_t e = launch_forward(P, ws, Rc, O, st, true);
+      if (e != hipSuccess) return e;
+      A.ray0 = a;
+      A.nr = c;
+      A.tri_stride = c;  // the chunk's forward wrote its list with one row of c slots per event index
+      A.order = O.ray_order;
+      backward_trace(A, st);
     }
   }
+  backward_end(P, ws, R, B, C, st);
   return hipGetLastError();
 }
 

@@ -200,30 +200,58 @@ int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspac
                     "ever_trace_forward");
 }
 
-int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
-                        const ever_backward_io* io, void* stream) {
+// Checks shared by both adjoints; `stored`: the event list (and, by slot, the ray order) comes from the caller.
+static int check_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
+                          const ever_backward_io* io, bool stored) {
   if (int rc = check_prims(prims)) return rc;
   if (int rc = check_rays(rays)) return rc;
   if (!io) return fail(EVER_ERR_INVALID, "io is null");
-  if (rays->event_layout == EVER_EVENTS_BY_SLOT && rays->num_rays > 0 && !io->ray_order)
+  if (stored && rays->event_layout == EVER_EVENTS_BY_SLOT && rays->num_rays > 0 && !io->ray_order)
     return fail(EVER_ERR_INVALID, "event lists stored by slot need the forward's ray_order");
   if (prims->num_views > 1 && prims->num_prims > 0 && rays->num_rays > 0 && (!rays->view_index || !io->view_color_grads))
     return fail(EVER_ERR_INVALID, "view_index and view_color_grads are needed when num_views > 1");
   if (!workspace) return fail(EVER_ERR_INVALID, "workspace is null");
   if (workspace_bytes < ever_scene_workspace_bytes(prims->num_prims))
     return fail(EVER_ERR_WORKSPACE, "scene workspace too small");
-  const int64_t nw = io->wcts ? io->num_wcts : 0;
   if (io->wcts && io->num_wcts < 1) return fail(EVER_ERR_INVALID, "num_wcts must be >= 1 when wcts is given");
   if (io->wcts && !io->dmeans2D) return fail(EVER_ERR_INVALID, "dmeans2D is null while wcts is given");
-  if (!io->workspace ||
-      io->workspace_bytes < ever_backward_workspace_bytes(prims->num_prims, rays->num_rays, nw))
-    return fail(EVER_ERR_WORKSPACE, "backward workspace too small");
   if (prims->num_prims > 0 && (!io->dmeans || !io->dscales || !io->dquats || !io->ddensities || !io->dfeatures))
     return fail(EVER_ERR_INVALID, "a primitive gradient output is null");
   if (rays->num_rays > 0 && (!io->grad_out || !io->iters || !io->states || !io->ray_state64 || !io->drayo ||
-                             !io->drayd || !io->tri_collection))
-    return fail(EVER_ERR_INVALID, "a ray input/output of the backward is null (stored events are required)");
+                             !io->drayd || (stored && !io->tri_collection)))
+    return fail(EVER_ERR_INVALID, stored ? "a ray input/output of the backward is null (stored events are required)"
+                                         : "a ray input/output of the backward is null");
+  return EVER_OK;
+}
+
+int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes, const ever_rays* rays,
+                        const ever_backward_io* io, void* stream) {
+  if (int rc = check_backward(prims, workspace, workspace_bytes, rays, io, true)) return rc;
+  const int64_t nw = io->wcts ? io->num_wcts : 0;
+  if (!io->workspace ||
+      io->workspace_bytes < ever_backward_workspace_bytes(prims->num_prims, rays->num_rays, nw))
+    return fail(EVER_ERR_WORKSPACE, "backward workspace too small");
   return hip_status(ever::launch_backward(*prims, workspace, *rays, *io, (hipStream_t)stream), "ever_trace_backward");
+}
+
+size_t ever_backward_recompute_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts,
+                                               int32_t max_iters, int32_t traversal, int64_t chunk_rays) {
+  if (num_prims < 0 || num_rays < 0 || num_wcts < 0 || max_iters < 0 || chunk_rays < 1) return 0;
+  return ever::recompute_layout(num_prims, num_rays, num_wcts, max_iters, traversal, chunk_rays).total;
+}
+
+int ever_trace_backward_recompute(const ever_prims* prims, void* workspace, size_t workspace_bytes,
+                                  const ever_rays* rays, const ever_backward_io* io, int64_t chunk_rays,
+                                  void* stream) {
+  if (int rc = check_backward(prims, workspace, workspace_bytes, rays, io, false)) return rc;
+  if (chunk_rays < 1) return fail(EVER_ERR_INVALID, "chunk_rays must be >= 1");
+  const int64_t nw = io->wcts ? io->num_wcts : 0;
+  if (!io->workspace || io->workspace_bytes < ever_backward_recompute_workspace_bytes(
+                                                  prims->num_prims, rays->num_rays, nw, rays->max_iters,
+                                                  rays->traversal, chunk_rays))
+    return fail(EVER_ERR_WORKSPACE, "backward workspace too small");
+  return hip_status(ever::launch_backward_recompute(*prims, workspace, *rays, *io, chunk_rays, (hipStream_t)stream),
+                    "ever_trace_backward_recompute");
 }
 
 int ever_scene_from_records(const float* records, int64_t num_prims, const ever_ply_layout* layout,

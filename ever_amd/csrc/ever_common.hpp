@@ -270,82 +270,77 @@ __device__ __forceinline__ double safe_depth(d3 ro, f3 rd, f3 s, float4 q, int f
   return face == 1 ? fmin(D.r0, D.r1) : fmax(D.r0, D.r1);
 }
 
-struct SafeRoots {
-  f3 Tro, Trd, rd_rot, ocn, rdn;
-  float a, bp, ba, omll, h, c, sh, sb, qq, r0, r1;
-  f3 l;
-  double t0, t1;  // (min, max) root in FP64
-};
-__device__ __forceinline__ bool safe_roots(d3 ro, f3 rd, f3 s, float4 q, SafeRoots& S) {
-  SafeRootsD D;
-  safe_roots_d(ro, rd, s, q, D);
-  S.rd_rot = tof(D.rd_rot);
-  S.Trd = tof(D.Trd);
-  S.Tro = tof(D.Tro);
-  S.ocn = tof(D.ocn);
-  S.rdn = tof(D.rdn);
-  S.a = (float)D.a;
-  S.bp = (float)D.bp;
-  S.ba = (float)D.ba;
-  S.l = tof(D.l);
-  S.omll = (float)D.omll;
-  S.h = (float)D.h;
-  S.c = (float)D.c;
-  if (!D.hit) {
-    S.r0 = S.r1 = -1.f;
-    S.t0 = S.t1 = -1.0;
-    return false;
-  }
-  S.sh = (float)D.sh;
-  S.sb = (float)D.sb;
-  S.qq = (float)D.qq;
-  S.r0 = (float)D.r0;
-  S.r1 = (float)D.r1;
-  S.t0 = fmin(D.r0, D.r1);
-  S.t1 = fmax(D.r0, D.r1);
-  return true;
+// FP64 forms of the safe-math adjoint helpers (safe-math.slang:68-106), for gradients only: reciprocals from the
+// hardware estimate (v_rcp_f64, v_rsq_f64) refined by two Newton steps, within a few ulp of the IEEE operations.
+__device__ __forceinline__ double clipd(double v) { return fmax(fmin(v, (double)kMaxVal), (double)kMinVal); }
+__device__ __forceinline__ double rcp_nr_d(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return fma(fma(-b, r, 1.0), r, r);
 }
-__device__ __forceinline__ double safe_root_select(const SafeRoots& S, int face) { return face == 1 ? S.t0 : S.t1; }
-// Adjoint of the selected root w.r.t. ro = origin - mean, rd = raw direction, scale and raw quaternion.
-__device__ __forceinline__ void safe_roots_bwd(const SafeRoots& S, f3 ro, f3 rd, f3 s, float4 q, int face,
-                                               float tb, f3& rob, f3& rdb, f3& sbar, float4& qb) {
-  // ro enters only through the rotation adjoint, which is linear in it
-  float r0b = 0.f, r1b = 0.f;
-  if (face == 1) {
-    if (S.r0 < S.r1) r0b = tb; else r1b = tb;
-  } else {
-    if (S.r0 > S.r1) r0b = tb; else r1b = tb;
+// 1/b' and 1/b2' of a guarded denominator: b' = b, or kTiny where |b| < kTiny; b2' = b*b, or kTiny below it
+struct SafeRcpD {
+  double r1, r2;
+};
+__device__ __forceinline__ SafeRcpD safe_rcp_fd(double b) {
+  const double r1 = rcp_nr_d(fabs(b) < (double)kTiny ? (double)kTiny : b);
+  return {r1, b * b < (double)kTiny ? 1.0 / (double)kTiny : r1 * r1};
+}
+
+// Adjoint of the selected safe root (tri-intersect.slang:286-344) w.r.t. ro = origin - mean, rd = raw direction,
+// scale and raw quaternion, in FP64 from the ray in the primitive's normalized frame (ocn = rotated origin / scale,
+// rdn = rotated unit direction / scale, both from safe_roots_d). Along a thin axis ocn and t * rdn are large and
+// nearly opposite (their sum is the hit point on the unit sphere), so the scale gradient -(ocn*ocnb + rdn*rdnb)/s
+// cancels by |ocn|: in FP32 that loses three to four digits, in FP64 none that matter. sel_r0: the selected root
+// is r0 = c / q (else r1 = q / a).
+__device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro, f3 rd, f3 s, float4 q, float tb,
+                                                 f3& rob, f3& rdb, f3& sbar, float4& qb) {
+  // the forward quantities, recomputed as safe_roots_d computes them
+  const double a = dotd(d, d);
+  const double bp = -dotd(o, d);
+  const SafeRcpD ra = safe_rcp_fd(a);
+  const double ba = safe_mul_d(bp, ra.r1);
+  const d3 l = o + ba * d;
+  const double omll = 1.0 - dotd(l, l);
+  const double h = a * omll;
+  const double c = dotd(o, o) - 1.0;
+  // sqrt(h) and 1/sqrt(h) from the reciprocal square root estimate and two Newton steps
+  double rs = 0.0, sh = 0.0;
+  if (!(h < (double)kTiny)) {
+    rs = __builtin_amdgcn_rsq(h);
+    rs = rs * fma(-0.5 * h * rs, rs, 1.5);
+    rs = rs * fma(-0.5 * h * rs, rs, 1.5);
+    sh = h * rs;
   }
-  float cb, qqb, ab = 0.f, bpb, hb = 0.f, da, db;
-  bw_safe_div(S.c, S.qq, r0b, da, db);
-  cb = da;
-  qqb = db;
-  const SafeRcp ra = safe_rcp(S.a);
-  bw_safe_div_r(S.qq, ra, r1b, da, db);
-  qqb += da;
-  ab += db;
-  bpb = qqb;
-  if (!(S.h < kTiny)) hb = qqb * S.sb * 0.5f / S.sh;
-  ab += hb * S.omll;
-  f3 lb = (2.f * (-hb * S.a)) * S.l;
-  f3 ocnb = (2.f * cb) * S.ocn + lb;
-  float bab = dot3(lb, S.rdn);
-  f3 rdnb = S.ba * lb;
-  bw_safe_div_r(S.bp, ra, bab, da, db);
-  bpb += da;
-  ab += db;
-  ocnb = ocnb + (-bpb) * S.rdn;
-  rdnb = rdnb + (-bpb) * S.ocn + (2.f * ab) * S.rdn;
-  f3 Trob, Trdb;
-  const SafeRcp rx = safe_rcp(s.x), ry = safe_rcp(s.y), rz = safe_rcp(s.z);
-  bw_safe_div_r(S.Tro.x, rx, ocnb.x, da, db); Trob.x = da; sbar.x += db;
-  bw_safe_div_r(S.Tro.y, ry, ocnb.y, da, db); Trob.y = da; sbar.y += db;
-  bw_safe_div_r(S.Tro.z, rz, ocnb.z, da, db); Trob.z = da; sbar.z += db;
-  bw_safe_div_r(S.Trd.x, rx, rdnb.x, da, db); Trdb.x = da; sbar.x += db;
-  bw_safe_div_r(S.Trd.y, ry, rdnb.y, da, db); Trdb.y = da; sbar.y += db;
-  bw_safe_div_r(S.Trd.z, rz, rdnb.z, da, db); Trdb.z = da; sbar.z += db;
+  const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
+  const double qq = bp + sb * sh;
+  const double r0b = sel_r0 ? (double)tb : 0.0, r1b = sel_r0 ? 0.0 : (double)tb;
+  // r0 = safe_div(c, qq), r1 = safe_div(qq, a)
+  const SafeRcpD rq = safe_rcp_fd(qq);
+  const double cb = clipd(r0b * rq.r1);
+  double qqb = clipd(-c * rq.r2 * r0b) + clipd(r1b * ra.r1);
+  double ab = clipd(-qq * ra.r2 * r1b);
+  const double hb = qqb * sb * 0.5 * rs;  // 0 where h < kTiny
+  ab += hb * omll;
+  const d3 lb = (-2.0 * hb * a) * l;
+  d3 ocnb = (2.0 * cb) * o + lb;
+  const double bab = dotd(lb, d);
+  d3 rdnb = ba * lb;
+  // ba = safe_div(bp, a)
+  const double bpb = qqb + clipd(bab * ra.r1);
+  ab += clipd(-bp * ra.r2 * bab);
+  ocnb = ocnb + (-bpb) * d;
+  rdnb = rdnb + (-bpb) * o + (2.0 * ab) * d;
+  // ocn = safe_div(Tro, s), rdn = safe_div(Trd, s), with Tro = ocn * s and Trd = rdn * s
+  const SafeRcpD rx = safe_rcp_fd(s.x), ry = safe_rcp_fd(s.y), rz = safe_rcp_fd(s.z);
+  const f3 Trob = mk3((float)clipd(ocnb.x * rx.r1), (float)clipd(ocnb.y * ry.r1), (float)clipd(ocnb.z * rz.r1));
+  const f3 Trdb = mk3((float)clipd(rdnb.x * rx.r1), (float)clipd(rdnb.y * ry.r1), (float)clipd(rdnb.z * rz.r1));
+  const double sx = (double)s.x, sy = (double)s.y, sz = (double)s.z;
+  sbar.x += (float)(clipd(-(o.x * sx) * rx.r2 * ocnb.x) + clipd(-(d.x * sx) * rx.r2 * rdnb.x));
+  sbar.y += (float)(clipd(-(o.y * sy) * ry.r2 * ocnb.y) + clipd(-(d.y * sy) * ry.r2 * rdnb.y));
+  sbar.z += (float)(clipd(-(o.z * sz) * rz.r2 * ocnb.z) + clipd(-(d.z * sz) * rz.r2 * rdnb.z));
   qrot_bwd(ro, q, Trob, rob, qb);
-  f3 rrb = l2n_bwd(S.rd_rot, Trdb);
+  const f3 rrb = l2n_bwd(qrot(rd, q), Trdb);
   qrot_bwd(rd, q, rrb, rdb, qb);
 }
 

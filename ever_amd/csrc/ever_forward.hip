@@ -1026,15 +1026,15 @@ static void (*pick_kernel(int kbuf, int flags))(FwdArgs) {
 }
 
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
-                          hipStream_t st) {
+                          hipStream_t st, bool keep_candidates) {
   SceneLayout L = scene_layout(P.num_prims);
   uint32_t* header = at<uint32_t>(ws, L.header);
   int32_t* cand = at<int32_t>(ws, L.cand);
-  hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, st, header);
+  if (!keep_candidates) hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, st, header);
   if (O.touch_count && P.num_prims > 0)
     hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, O.touch_count, P.num_prims);
   if (R.num_rays <= 0) return hipGetLastError();
-  if (P.num_prims > 0)
+  if (P.num_prims > 0 && !keep_candidates)
     hipLaunchKernelGGL(k_prefilter, dim3(grid_for(P.num_prims, 256)), dim3(256), 0, st, P, R.origins, R.tmin, cand,
                        header);
   // processing order

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stddef.h>
 #include <stdint.h>
 #include <string>
@@ -121,10 +122,15 @@ hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t s);
 // Rewrites the per-primitive colours of a built scene from new features (geometry and tree untouched).
 hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t s);
+// keep_candidates: trace with the start-inside candidate list an earlier forward call left in the scene workspace
+// (the rays are a part of that call's rays, whose first ray chose the candidates) instead of building it again
 hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, const ever_forward_out& O,
-                          hipStream_t s);
+                          hipStream_t s, bool keep_candidates = false);
 hipError_t launch_backward(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
                            hipStream_t s);
+// The adjoint without stored event lists: the rays are traced again chunk by chunk into one chunk-sized list
+hipError_t launch_backward_recompute(const ever_prims& P, void* ws, const ever_rays& R, const ever_backward_io& B,
+                                     int64_t chunk, hipStream_t s);
 struct MortonLayout {
   size_t box, part, keys0, vals0, keys1, vals1, hi, hist, total;
 };
@@ -208,6 +214,35 @@ inline ForwardLayout forward_layout(int64_t nr, bool per_ray) {
   L.hist = take(4 * (size_t)kRadix * (L.tiles + 1));  // + digit totals
   L.counter = take(4);                                 // next tile of a persistent launch
   L.ist = take(112 * (size_t)r);
+  L.total = off;
+  return L;
+}
+
+// Backward workspace of the recomputing adjoint: the backward's own regions, then one chunk's forward workspace,
+// event list (slot-major, chunk x max_iters) and forward outputs, reused by every chunk.
+struct RecomputeLayout {
+  int64_t chunk = 0;
+  size_t fwd, tri, color, states, iters, init, st64, order, total;
+};
+inline RecomputeLayout recompute_layout(int64_t n, int64_t nr, int64_t nw, int max_iters, int traversal,
+                                        int64_t chunk) {
+  RecomputeLayout L;
+  L.chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, nr));
+  const size_t c = (size_t)L.chunk;
+  size_t off = align_up(backward_layout(n, nr, nw).total, 256);
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  L.fwd = take(forward_layout(L.chunk, per_ray_walk(L.chunk, traversal)).total);
+  L.tri = take(4 * c * (size_t)std::max(max_iters, 0));
+  L.color = take(20 * c);
+  L.states = take(64 * c);
+  L.iters = take(4 * c);
+  L.init = take(16 * c);
+  L.st64 = take(64 * c);
+  L.order = take(4 * c);
   L.total = off;
   return L;
 }

@@ -25,16 +25,24 @@ from . import sp  # noqa: F401  (the reference's compiled module of the same nam
 # library's per-primitive epilogue and in the per-ray adjoint kernel.
 
 # Adjoint event source. "stored": the forward keeps every ray's event list (R * max_iters int32, the reference's
-# tri_collection) for the backward. "recompute": the forward keeps none; the backward traces the batch again
-# (same scene workspace, same deterministic walk, hence the same events) into a transient list, so nothing of
-# that size is held between forward and backward. Results are identical; "recompute" costs one more forward.
+# tri_collection) for the backward. "recompute": the forward keeps none; the backward traces the rays again, a
+# chunk at a time (same scene workspace, same deterministic walk, hence the same events), into one list of
+# chunk * max_iters ids, so no list of size R * max_iters exists at any time. Gradients are equal up to the order
+# of float atomic additions; "recompute" costs one more forward.
 _EVENT_MODE = "stored"
+_CHUNK_RAYS = T.RECOMPUTE_CHUNK
 
 
-def set_event_mode(mode):
-    global _EVENT_MODE
+def set_event_mode(mode, chunk_rays=None):
+    """Select "stored" or "recompute" event lists; chunk_rays sets the rays traced per chunk of the recomputing
+    adjoint (its event list holds chunk_rays * max_iters ids)."""
+    global _EVENT_MODE, _CHUNK_RAYS
     if mode not in ("stored", "recompute"):
         raise ValueError("event mode must be 'stored' or 'recompute'")
+    if chunk_rays is not None:
+        if int(chunk_rays) < 1:
+            raise ValueError("chunk_rays must be >= 1")
+        _CHUNK_RAYS = int(chunk_rays)
     _EVENT_MODE = mode
 
 
@@ -66,6 +74,8 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
     ctx.event_layout = layout
     ctx.recompute = recompute
     ctx.proposal = proposal
+    ctx.rays = out["rays"]
+    ctx.chunk_rays = _CHUNK_RAYS
     ctx.scene = scene
     ctx.max_iters = max_iters
     ctx.max_prim_size = max_prim_size
@@ -100,16 +110,12 @@ def _run_forward(ctx, mean, scale, quat, density, color, rayo, rayd, tmin, tmax,
 def _run_backward(ctx, grad_output):
     (mean, scale, quat, density, features, rayo, rayd, tri_collection, wcts, initial_drgb, states, iters, st64,
      order, _ws) = ctx.saved_tensors
-    if ctx.recompute:
-        # trace again for the event lists (scene workspace kept from the forward; same walk, same events)
-        again = T.trace_forward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, store_events=True,
-                                kbuf=ctx.kbuf, proposal=ctx.proposal, count_touches=False,
-                                event_layout=ctx.event_layout)
-        tri_collection = again["tri_collection"]
+    # recompute: no event list was kept (tri_collection is None) and the adjoint traces the rays again in chunks
     fwd = dict(tri_collection=tri_collection, states=states, iters=iters, ray_state64=st64,
-               initial_drgb=initial_drgb, ray_order=order, event_layout=ctx.event_layout)
+               initial_drgb=initial_drgb, ray_order=order, event_layout=ctx.event_layout, rays=ctx.rays,
+               proposal=ctx.proposal)
     g = T.trace_backward(ctx.scene, rayo, rayd, ctx.tmin, ctx.tmax, ctx.max_iters, fwd, grad_output, wcts=wcts,
-                         kbuf=ctx.kbuf)
+                         kbuf=ctx.kbuf, chunk_rays=ctx.chunk_rays)
     return (g["dmeans"], g["dscales"], g["dquats"], g["ddensities"].reshape(density.shape), g["dfeatures"],
             g["drayo"], g["drayd"], g.get("dmeans2D"))
 

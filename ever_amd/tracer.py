@@ -269,11 +269,20 @@ def trace_forward(scene, rayo, rayd, tmin, tmax, max_iters, store_events=True, z
         _t1(tok)
     return dict(color=color, states=states, iters=iters, tri_collection=tri, touch_count=touch, initial_drgb=init,
                 ray_state64=st64, ray_order=order, rays=rays, view_index=view_index, event_layout=event_layout,
-                last_dirac=dirac)
+                last_dirac=dirac, proposal=proposal)
 
 
-def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None):
-    """Adjoint pass (SplineTracer.backward, fast_ellipsoid_splinetracer.py:105-219). Returns a dict of grads."""
+# rays per chunk of the recomputing adjoint: its event list holds chunk x max_iters ids (512 MB at 500 events)
+RECOMPUTE_CHUNK = 262144
+
+
+def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts=None, kbuf=None,
+                   chunk_rays=None):
+    """Adjoint pass (SplineTracer.backward, fast_ellipsoid_splinetracer.py:105-219). Returns a dict of grads.
+
+    When the forward kept no event list (fwd["tri_collection"] is None) the rays are traced again, chunk_rays
+    (default RECOMPUTE_CHUNK) at a time, into one chunk-sized list (ever_trace_backward_recompute); the
+    proposal, view index and traversal of fwd["rays"] are reused."""
     dev = scene.device
     n = scene.n
     nr = rayo.numel() // 3
@@ -296,11 +305,19 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
         _check_float(wcts, "wcts", 4, dev)
         nw = wcts.numel() // 16
         out["dmeans2D"] = torch.empty((n, 2), dtype=torch.float32, device=dev)
-    ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     vi = fwd.get("view_index")
-    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, 0, None,
-                  C.ptr(vi), 0, EVENT_LAYOUTS[fwd.get("event_layout", "ray")])
+    recompute = fwd.get("tri_collection") is None
+    frays = fwd.get("rays")
+    traversal = frays.traversal if frays is not None else TRAVERSALS[_TRAVERSAL]
+    chunk = int(chunk_rays or RECOMPUTE_CHUNK)
+    if recompute:
+        ws_bytes = L.ever_backward_recompute_workspace_bytes(n, nr, nw, int(max_iters), traversal, chunk)
+    else:
+        ws_bytes = L.ever_backward_workspace_bytes(n, nr, nw)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    rays = C.Rays(C.ptr(rayo), C.ptr(rayd), nr, float(tmin), float(tmax), int(max_iters), kbuf or _KBUF, traversal,
+                  C.ptr(fwd.get("proposal")) if recompute else None, C.ptr(vi), 0,
+                  EVENT_LAYOUTS[fwd.get("event_layout", "ray")])
     vgrad = None
     if scene.num_views > 1:
         vgrad = torch.empty((scene.num_views, n, 4), dtype=torch.float32, device=dev)
@@ -312,7 +329,12 @@ def trace_backward(scene, rayo, rayd, tmin, tmax, max_iters, fwd, grad_out, wcts
                       ws_bytes, C.ptr(vgrad))
     with torch.cuda.device(dev):
         tok = _t0("backward")
-        C.check(L.ever_trace_backward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
-                                      ctypes.byref(rays), ctypes.byref(io), C.stream_ptr()), "ever_trace_backward")
+        if recompute:
+            C.check(L.ever_trace_backward_recompute(ctypes.byref(scene.prims), C.ptr(scene.workspace),
+                                                    scene.ws_bytes, ctypes.byref(rays), ctypes.byref(io), chunk,
+                                                    C.stream_ptr()), "ever_trace_backward_recompute")
+        else:
+            C.check(L.ever_trace_backward(ctypes.byref(scene.prims), C.ptr(scene.workspace), scene.ws_bytes,
+                                          ctypes.byref(rays), ctypes.byref(io), C.stream_ptr()), "ever_trace_backward")
         _t1(tok)
     return out

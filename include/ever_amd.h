@@ -18,6 +18,7 @@
  *   ever_trace_backward   splinetracers/fast_ellipsoid_splinetracer.py:105-219 (SplineTracer.backward) ->
  *                         slang/backwards_kernel.slang:259-367 (backwards_kernel), :374-428
  *                         (backwards_initial_drgb_kernel), gradient clamps fast_ellipsoid_splinetracer.py:201-211
+ *   ever_trace_backward_recompute  the same adjoint without the stored event list of py_binding.cpp:221-223
  *   ever_eval_sh          eval_sh.py:31-55 (EvalSH.forward) -> splinetracers/slang/sh_kernel.slang:19-42
  *   ever_eval_sh_backward eval_sh.py:57-68 (EvalSH.backward) -> splinetracers/slang/sh_kernel.slang:45-98
  *   ever_scene_from_records  splinetracers/fast_ellipsoid_splinetracer/ply_file_loader.cc:150-293 (ReadSceneFromFile,
@@ -37,7 +38,7 @@
 extern "C" {
 #endif
 
-#define EVER_ABI_VERSION 8
+#define EVER_ABI_VERSION 9
 
 /* status codes */
 #define EVER_OK 0
@@ -180,6 +181,21 @@ int ever_trace_forward(const ever_prims* prims, void* workspace, size_t workspac
 size_t ever_backward_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts);
 int ever_trace_backward(const ever_prims* prims, void* workspace, size_t workspace_bytes,
                         const ever_rays* rays, const ever_backward_io* io, void* stream);
+
+/* The adjoint without a stored event list. Replaces the R x max_iters tri_collection that Forward.trace_rays
+ * allocates and keeps for the backward (py_binding.cpp:221-223, read back at backwards_kernel.slang:327-361): the
+ * forward is called with tri_collection = NULL, and this call traces the rays again, chunk_rays at a time, into one
+ * chunk-sized event list inside its workspace, running the adjoint on each chunk before the next. Same inputs and
+ * outputs as ever_trace_backward except that io->tri_collection and io->ray_order are ignored; rays must be those of
+ * the forward call, and the scene workspace must still hold that call's start-inside candidates (no forward on it in
+ * between, as for ever_trace_backward). Gradients equal the stored-list adjoint's up to the order of float atomic
+ * additions. The workspace, ever_backward_recompute_workspace_bytes(...) bytes, grows with chunk_rays x max_iters
+ * instead of R x max_iters. */
+size_t ever_backward_recompute_workspace_bytes(int64_t num_prims, int64_t num_rays, int64_t num_wcts,
+                                               int32_t max_iters, int32_t traversal, int64_t chunk_rays);
+int ever_trace_backward_recompute(const ever_prims* prims, void* workspace, size_t workspace_bytes,
+                                  const ever_rays* rays, const ever_backward_io* io, int64_t chunk_rays,
+                                  void* stream);
 
 /* Per-primitive SH colour at normalize(mean - origin), unclamped (sh_kernel.slang:19-42). */
 int ever_eval_sh(const float* means, const float* features, const float* origin, int64_t num_prims,

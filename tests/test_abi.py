@@ -28,7 +28,7 @@ def test_every_header_function_is_exported_and_bound():
 
 def test_abi_version_and_struct_sizes():
     L = C.lib()
-    assert L.ever_abi_version() == 8
+    assert L.ever_abi_version() == 9
     sizes = (ctypes.c_int64 * 4)()
     assert L.ever_struct_sizes(sizes) == 0
     assert list(sizes) == [ctypes.sizeof(C.Prims), ctypes.sizeof(C.Rays), ctypes.sizeof(C.ForwardOut),
@@ -47,6 +47,13 @@ def test_workspace_queries_grow_with_size():
         ray, wave = L.ever_forward_workspace_bytes_for(nr, 1), L.ever_forward_workspace_bytes_for(nr, 2)
         assert ray - wave >= 4 * 150 * nr
         assert L.ever_forward_workspace_bytes(nr) == (ray if nr < 65536 else wave)
+    # the recomputing adjoint holds one chunk's event list (chunk x max_iters ids), not the call's
+    n, nr, it = 100000, 2_000_000, 500
+    stored = L.ever_backward_workspace_bytes(n, nr, 0) + 4 * nr * it
+    chunk = L.ever_backward_recompute_workspace_bytes(n, nr, 0, it, 0, 131072)
+    assert chunk < L.ever_backward_recompute_workspace_bytes(n, nr, 0, it, 0, 262144) < stored - 3e9
+    assert L.ever_backward_workspace_bytes(n, nr, 0) + 4 * 131072 * it <= chunk < stored - 3.5e9
+    assert L.ever_backward_recompute_workspace_bytes(n, nr, 0, it, 0, 0) == 0
     lay = C.scene_layout(1000)
     offs = [lay[k] for k in C.LAYOUT_NAMES]
     assert offs == sorted(offs) and lay["total"] == a
@@ -111,6 +118,9 @@ def test_invalid_arguments_return_status_codes():
                                        None) == 1
     assert b"EVER_MORTON_NOALLOC_MAX_PRIMS" in L.ever_last_error()
     assert L.ever_morton_order_noalloc(None, 0, None, None) == 0
+    # the recomputing adjoint checks its arguments as ever_trace_backward does, without an event list
+    assert L.ever_trace_backward_recompute(None, None, 0, None, None, 1024, None) == 1
+    assert b"prims is null" in L.ever_last_error()
     cam = C.Camera()
     assert L.ever_camera_rays(None, None, None, None, None) == 1
     cam.width, cam.height, cam.fx, cam.fy = 4, 3, 1.0, 1.0

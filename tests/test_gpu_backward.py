@@ -128,9 +128,9 @@ def test_clustered_subset_gradients():
     ref = oracle.forward(*args, o.double().numpy()[idx], d.double().numpy()[idx], 0.0, 100.0, use_bvh=True)
     rb = oracle.backward(*args, o.double().numpy()[idx], d.double().numpy()[idx], 0.0, 500, g[idx], ref)
     for k in ("dmeans", "dscales", "dquats", "ddensities", "dfeatures"):
-        # the scale gradient along the thin axis of needle-like primitives is ill-conditioned: the same algorithm in
-        # float32 is off by 3.9e-3 (L2) and 8.4e-3 (max) here; the L2 bar for it is 2e-4
-        assert_grad_close(_np(b[k]), rb[k], k, l2=2e-4 if k == "dscales" else 1e-4)
+        # the scale gradient along the thin axis of needle-like primitives is ill-conditioned (the same algorithm
+        # in float32 is off by 3.9e-3 L2 and 8.4e-3 max here); the root adjoint is evaluated in FP64 for it
+        assert_grad_close(_np(b[k]), rb[k], k)
     for k in ("drayo", "drayd"):
         assert_grad_close(_np(b[k])[idx], rb[k], k)
 

@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from ever_amd import scenes
+from ever_amd import tracer as T
 from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
 from oracle import oracle
 
@@ -29,8 +30,11 @@ def _grads(params, o, d, g):
     return out.detach(), gr
 
 
-def test_recomputed_events_equal_stored_events():
-    print("case: clustered 100k, 320x180, stored vs recomputed events", flush=True)
+@pytest.mark.parametrize("chunk", [10_000, 65_536, 262_144])
+def test_recomputed_events_equal_stored_events(chunk):
+    # 57,600 rays: six chunks with a ragged last one (per-ray walk), one chunk of 65,536 (per-wave walk, sorted),
+    # one chunk larger than the call
+    print(f"case: clustered 100k, 320x180, stored vs recomputed events, chunks of {chunk} rays", flush=True)
     sc = scenes.clustered_scene(100_000, 3, seed=9)
     o, d = scenes.pinhole_rays(320, 180, EYE)
     o, d = o.to(D).contiguous(), d.to(D).contiguous()
@@ -39,11 +43,43 @@ def test_recomputed_events_equal_stored_events():
     try:
         fes.set_event_mode("stored")
         a, ga = _grads(params, o, d, g)
-        fes.set_event_mode("recompute")
+        fes.set_event_mode("recompute", chunk_rays=chunk)
         b, gb = _grads(params, o, d, g)
     finally:
-        fes.set_event_mode("stored")
+        fes.set_event_mode("stored", chunk_rays=T.RECOMPUTE_CHUNK)
     assert torch.equal(a, b)
+    for x, y in zip(ga, gb):
+        assert float((x - y).norm()) <= 1e-6 * float(x.norm()) + 1e-12
+
+
+def _peak_step(params, o, d, g):
+    """Peak device memory allocated by one forward + backward above what was allocated before it."""
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    _, gr = _grads(params, o, d, g)
+    torch.cuda.synchronize()
+    return torch.cuda.max_memory_allocated() - base, gr
+
+
+def test_recompute_bounds_memory_at_full_size():
+    """1M clustered SH3 primitives at 1920x1080 (2,073,600 rays, max_iters 500): the stored event list is 4.15 GB;
+    the recomputing adjoint holds one chunk's list instead, with the same gradients."""
+    print("case: C3 1M clustered 1920x1080, peak memory stored vs recompute", flush=True)
+    sc, o, d = scenes.config3()
+    o, d = o.to(D).contiguous(), d.to(D).contiguous()
+    g = scenes.upstream_grad(o.shape[0], seed=5).to(D)
+    params = _params(sc)
+    try:
+        fes.set_event_mode("stored")
+        peak_s, ga = _peak_step(params, o, d, g)
+        fes.set_event_mode("recompute", chunk_rays=T.RECOMPUTE_CHUNK)
+        peak_r, gb = _peak_step(params, o, d, g)
+    finally:
+        fes.set_event_mode("stored")
+    print(f"peak above baseline: stored {peak_s / 1e9:.3f} GB, recompute {peak_r / 1e9:.3f} GB, "
+          f"saved {(peak_s - peak_r) / 1e9:.3f} GB", flush=True)
+    assert peak_s - peak_r >= 3.5e9
     for x, y in zip(ga, gb):
         assert float((x - y).norm()) <= 1e-6 * float(x.norm()) + 1e-12
 
@@ -118,11 +154,14 @@ def test_gradients_match_finite_differences_on_256_parameters():
         an.append(grads[k][p].reshape(-1)[j])
     fd, fd2, an = np.array(fd), np.array(fd2), np.array(an)
     scale = np.median(np.abs(fd)) + 1e-12
-    # a parameter on a kink (grazing silhouette, two events changing order) has no derivative there: the two
-    # step sizes disagree, and such parameters are reported and left out
-    smooth = np.abs(fd - fd2) <= 1e-2 * np.maximum(np.abs(fd), scale)
+    # central differences err by c*h^2: fd2 - fd = 3*c*h^2 measures it and Richardson's (4*fd - fd2)/3 removes it.
+    # A parameter on a kink (grazing silhouette, two events changing order) has no derivative there, and one next
+    # to a kink has a curvature the steps cannot resolve: the two step sizes disagree by more than 1e-3, and such
+    # parameters are reported and left out
+    smooth = np.abs(fd - fd2) <= 1e-3 * np.maximum(np.abs(fd), scale)
     smooth &= np.abs(an) < 999.0  # gradients clamped at +-1e3 (fast_ellipsoid_splinetracer.py:201-211)
     kinds = np.array([k for k, _, _ in picks])
+    fd = (4.0 * fd - fd2) / 3.0
     err = np.abs(fd - an) / np.maximum(np.abs(fd), scale)
     print("FD rel err (smooth params): median", np.median(err[smooth]), "p95", np.percentile(err[smooth], 95),
           "max", err[smooth].max(), "; left out", int((~smooth).sum()), "of", len(fd), flush=True)
@@ -130,5 +169,5 @@ def test_gradients_match_finite_differences_on_256_parameters():
         sel = smooth & (kinds == k)
         if sel.any():
             print("  ", names[k], int(sel.sum()), "median", np.median(err[sel]), "max", err[sel].max(), flush=True)
-    assert smooth.sum() >= 200
-    assert np.median(err[smooth]) < 1e-4 and np.percentile(err[smooth], 95) < 1e-3
+    assert smooth.sum() >= 240
+    assert np.median(err[smooth]) < 1e-4 and err[smooth].max() <= 1e-3

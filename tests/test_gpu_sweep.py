@@ -1,9 +1,10 @@
 """Randomised parity sweep of the HIP tracer against the FP64 oracle: seeded draws of scene kind and size,
 feature degree, camera (outside or inside the scene), tmin, tmax, max_iters, k-buffer size and tree walk, each
 checked forward (colour and opacity 1e-4 absolute, distortion 1e-4 relative, per-ray event counts exact) and
-backward (every gradient tensor: 1e-3 relative L2 error; the largest single error 1e-3 of the tensor's largest
-element, raised up to 1e-2 where the same algorithm evaluated in float32 is itself off by more than half that,
-i.e. scale gradients along the thin axis of needle-like primitives).
+backward (every gradient tensor: 1e-3 relative L2 error, and the largest single error 1e-3 of the tensor's
+largest element). The same algorithm evaluated in float32 on the CPU is printed beside each figure: it shows how
+far the case is conditioned at float32 precision (it misses these bars on scale gradients along the thin axis of
+needle-like primitives, which the tracer's FP64 root adjoint holds).
 Cases with a tmax that cuts through primitives and k-buffer sizes other than 16 check that results follow the
 reference's 16-event rounds whatever the buffer size."""
 import math
@@ -103,7 +104,7 @@ def test_random_configuration_matches_oracle(seed):
                              traversal=c["traversal"], proposal=prop.to(D))
         assert torch.equal(fp["color"], f["color"]) and torch.equal(fp["iters"], f["iters"])
     # the same algorithm evaluated in float32 on the CPU: its distance to the FP64 oracle is the conditioning of
-    # the case at float32 precision, which no float32 implementation (the reference's included) goes below
+    # the case at float32 precision (reported, not used in the bars)
     r32 = oracle.forward(*npsc, o.double().numpy(), d.double().numpy(), c["tmin"], c["tmax"],
                          max_iters=c["max_iters"], use_bvh=True, dtype=np.float32)
     col = f["color"].cpu().double().numpy()
@@ -132,4 +133,4 @@ def test_random_configuration_matches_oracle(seed):
         f_mx = np.abs(r32k - r).max() / max(np.abs(r).max(), 1e-12)
         print(f"  {k}: L2 rel {e_l2:.2e}, max rel {e_mx:.2e} (float32 restatement: {f_l2:.2e}, {f_mx:.2e})",
               flush=True)
-        assert e_l2 <= 1e-3 and e_mx <= min(max(1e-3, 2.0 * f_mx), 1e-2), k
+        assert e_l2 <= 1e-3 and e_mx <= 1e-3, k
