@@ -503,7 +503,6 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
   ist_load(A, S.slot, I);
   d3 od, dd;  // the ray in FP64
   ray_fp64(A, S.r, od, dd);
-  const f3 rdraw = ld3(A.rayd + 3 * (int64_t)S.r);
   uint32_t last = 0xffffffffu;  // key of the last event integrated in this batch
   uint64_t ph = 0;              // the held-back event, its FP64 depth and its round condition (below)
   double pt = 0.0, pt0 = 0.0;
@@ -583,10 +582,11 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
     const float Df = (float)I.D0;
     const float area = fmaxf(Df * dt, 0.f);
     {
-      // the adjoint's depth of this event and its area, as ever_backward evaluates them
-      const float4 q = reinterpret_cast<const float4*>(A.quats)[s];
-      const double ts = safe_depth(od - tod(ld3(A.means + 3 * (int64_t)s)), rdraw, ld3(A.scales + 3 * (int64_t)s),
-                                   q, entry ? 1 : 0);
+      // the adjoint's depth of this event and its area, as ever_backward evaluates them: its safe roots measure
+      // depth along the rotated direction normalized after the rotation, which the raw quaternion scales, so the
+      // safe depth is the raw one times |qrot(d, q)| (the same point of the same line; FP64, within a few ulp)
+      const d3 rd = qrot_d(dd, reinterpret_cast<const float4*>(A.quats)[s]);
+      const double ts = t * sqrt(dotd(rd, rd));
       const float dts = fmaxf((float)(ts - I.ts_state), 0.f);
       I.logT_s = fmax((double)fmaxf(Df * dts, 0.f) + I.logT_s, 0.0);
       I.ts_state = ts;
