@@ -864,10 +864,10 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
         const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
         const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
-        // wave-uniform sort keys: the first live lane's entry distance (+inf where it misses) with the slot in
-        // the low 2 bits; children no lane enters sort last
-        uint32_t key[4];
-        int nhit = 0;
+        // the children some lane enters, as a wave-uniform bit mask, and each child's entry distance per lane
+        float tns[4];
+        uint32_t hmask = 0;
+        uint64_t hms[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           float tn, tf;
@@ -883,16 +883,31 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
             tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t_lo));
             tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
           }
-          // the lanes entering the box as one mask (ordered compare: a NaN bound enters nothing); the leader's
-          // entry distance for the order of the children, +inf where the leader does not enter
-          const uint64_t hm = __builtin_amdgcn_fcmpf(tn, tf, 5 /* ordered <= */);
-          const bool any = used[k] && hm != 0;
-          const uint32_t tl = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tn), leader);
-          const uint32_t lk = ((hm >> leader) & 1u) ? tl : 0x7f800000u;
-          key[k] = any ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
-          nhit += any ? 1 : 0;
+          // the lanes entering the box (ordered compare: a NaN bound enters nothing)
+          hms[k] = __builtin_amdgcn_fcmpf(tn, tf, 5 /* ordered <= */);
+          tns[k] = tn;
+          hmask |= (used[k] && hms[k] != 0) ? (1u << k) : 0u;
         }
-        if (nhit > 0) {
+        auto child = [&](uint32_t kk) {
+          const bool odd = (kk & 1u) != 0u;
+          const int32_t lo = odd ? c1 : c0, hi = odd ? c3 : c2;
+          return (kk & 2u) ? hi : lo;
+        };
+        if (hmask != 0) {
+          if ((hmask & (hmask - 1u)) == 0u) {  // one child entered: nothing to order or push
+            node = child((uint32_t)__builtin_ctz(hmask));
+            continue;
+          }
+          // wave-uniform sort keys: the first live lane's entry distance (+inf where it does not enter) with the
+          // slot in the low 2 bits; children no lane enters sort last
+          uint32_t key[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t tl = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tns[k]), leader);
+            const uint32_t lk = ((hms[k] >> leader) & 1u) ? tl : 0x7f800000u;
+            key[k] = ((hmask >> k) & 1u) ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
+          }
+          const int nhit = __builtin_popcount(hmask);
 #define EVER_CSWAP(a, b)                           \
   {                                                \
     const uint32_t lo_ = min(key[a], key[b]);      \
@@ -901,11 +916,6 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
   }
           EVER_CSWAP(0, 1) EVER_CSWAP(2, 3) EVER_CSWAP(0, 2) EVER_CSWAP(1, 3) EVER_CSWAP(1, 2)
 #undef EVER_CSWAP
-          auto child = [&](uint32_t kk) {
-            const bool odd = (kk & 1u) != 0u;
-            const int32_t lo = odd ? c1 : c0, hi = odd ? c3 : c2;
-            return (kk & 2u) ? hi : lo;
-          };
 #pragma unroll
           for (int k = 3; k >= 1; --k) {
             if (k < nhit) {  // sp < kStackBound <= 3 * 64 (tree bound, ever_internal.hpp)
