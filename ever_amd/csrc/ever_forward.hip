@@ -22,9 +22,8 @@ constexpr int kLdsDepth = kLdsStackPerRay;  // LDS stack entries per lane
 constexpr int kSpillDepth = kSpillPerRay;    // global spill entries per ray
 constexpr int kWaveBlock = 64;   // one wave per block in the per-wave traversal
 
-// Waves per SIMD that the per-wave walk is compiled for: 5 where the block's LDS (K events of 8 B, a 4-B
-// prefetch sink and a 4-B stack row per lane) lets more than 16 blocks share a CU's 160 KB, else 4. At K = 16, 18
-// blocks fit; at 96 VGPRs (20 spilled) the walk is 4.5% faster than at 127 VGPRs and 4 waves.
+// Waves per SIMD that the per-wave walk is compiled for (the VGPR budget: 128 at 4 waves). The block's LDS (K events
+// of 8 B and a 4-B stack row per lane) admits 18 blocks per CU at K = 16 and 16 at K = 19.
 template <int K>
 constexpr int wave_walk_waves() { return 4; }
 
@@ -817,7 +816,7 @@ __device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int 
 
 template <int K, bool kSigns, bool kBounded, bool kViews>
 __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int lane, uint32_t sg,
-                                           uint64_t (*ev)[kWaveBlock], uint32_t* pf_scratch, int32_t* row2) {
+                                           uint64_t (*ev)[kWaveBlock], int32_t* row2) {
   const int64_t nleaves = A.header[9];
   const int32_t root = nleaves == 1 ? ~0 : 0;
   bool done = false;  // (a lane without a ray is never alive: ray_idle)
@@ -924,14 +923,6 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
               else if (sp < 128) stk1 = (lane == sp - 64) ? ck : stk1;
               else if (lane == sp - 128) row2[lane] = ck;
               ++sp;
-              // bring the record into L2 before it is popped: an asynchronous copy into a scratch area of LDS
-              // that nothing reads
-              {
-                const float* src = ck >= 0 ? (const float*)(A.nodes4 + 8 * (int64_t)ck) + (lane & 31)
-                                           : (const float*)(A.rec + 4 * (int64_t)(uint32_t)(~ck)) + (lane & 15);
-                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                                 (__attribute__((address_space(3))) void*)pf_scratch, 4, 0, 0);
-              }
             }
           }
           node = child(key[0]);
@@ -966,7 +957,6 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
   constexpr bool kBounded = (kFlags & 1) != 0, kViews = (kFlags & 2) != 0, kPersistent = (kFlags & 4) != 0;
   const int lane = threadIdx.x;
   __shared__ uint64_t ev[K][kWaveBlock];
-  __shared__ uint32_t pf_scratch[kWaveBlock];
   __shared__ int32_t row2[kWaveBlock];  // third row of the wave's traversal stack
   // the 64 rays of processing slots [base, base + 64); lanes past the batch stay in the wave without a ray
   // (ray_idle), so that all 64 lanes hold the wave's stack
@@ -986,8 +976,8 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
     uint32_t sg = (signbit(inv0.x) ? 1u : 0u) | (signbit(inv0.y) ? 2u : 0u) | (signbit(inv0.z) ? 4u : 0u);
     const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
     if (!valid) sg = sg0;
-    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch, row2);
-    else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, pf_scratch, row2);
+    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, row2);
+    else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, row2);
     if (base + lane < A.nr) ray_write(A, S);
   };
   if (!kPersistent) {
