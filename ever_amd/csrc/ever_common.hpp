@@ -175,13 +175,7 @@ __device__ __forceinline__ d3 crossd(d3 a, d3 b) {
 }
 __device__ __forceinline__ d3 tod(f3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
 __device__ __forceinline__ f3 tof(d3 a) { return {(float)a.x, (float)a.y, (float)a.z}; }
-__device__ __forceinline__ double safe_div_d(double a, double b) {
-  const double r = a / (fabs(b) < (double)kTiny ? (double)kTiny : b);
-  return fmax(fmin(r, (double)kMaxVal), (double)kMinVal);
-}
-// safe_div_d by way of the (correctly rounded) reciprocal of the guarded denominator, for a denominator shared by
-// several quotients: within 1 ulp (FP64) of safe_div_d
-__device__ __forceinline__ double safe_rcp_d(double b) { return 1.0 / (fabs(b) < (double)kTiny ? (double)kTiny : b); }
+// safe_div(a, b) in FP64 as a * (1 / b'), b' the guarded denominator, clamped as safe_div clamps
 __device__ __forceinline__ double safe_mul_d(double a, double rb) {
   return fmax(fmin(a * rb, (double)kMaxVal), (double)kMinVal);
 }
@@ -191,40 +185,50 @@ __device__ __forceinline__ d3 qrot_d(d3 v, float4 q) {
   return v + (double)q.x * t + crossd(u, t);
 }
 
-// Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344): the rotated raw
-// direction is l2-normalized, divisions use safe_div. Evaluated in FP64 (so that 1 - |l|^2 and the root depths
-// keep their precision at grazing incidence and far from the ray origin) and kept in FP32 for the adjoint.
-// The FP64 part, shared by the backward and by the forward's bookkeeping of the same depths (both must see
-// identical values: the forward accumulates the log-transmittance the backward later unwinds event by event).
+// FP64 reciprocal and reciprocal square root from the hardware estimates (v_rcp_f64, v_rsq_f64) refined by two
+// Newton steps: within a few ulp of the IEEE operations.
+__device__ __forceinline__ double rcp_nr_d(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return fma(fma(-b, r, 1.0), r, r);
+}
+__device__ __forceinline__ double rsq_nr_d(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = r * fma(-0.5 * x * r, r, 1.5);
+  return r * fma(-0.5 * x * r, r, 1.5);
+}
+// 1 / b' of safe_div's guarded denominator (b' = b, or kTiny where |b| < kTiny)
+__device__ __forceinline__ double safe_rcp_fast_d(double b) { return rcp_nr_d(fabs(b) < (double)kTiny ? (double)kTiny : b); }
+
+// Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344): the rotated raw direction is
+// l2-normalized, divisions use safe_div. Evaluated in FP64, so that 1 - |l|^2 and the root depths keep their
+// precision at grazing incidence and far from the ray origin. The adjoint of the selected root
+// (safe_roots_bwd_d) starts from ocn and rdn.
 struct SafeRootsD {
-  d3 rd_rot, Trd, Tro, ocn, rdn, l;
-  double a, bp, ra, ba, omll, h, c, sh, sb, qq, r0, r1;
+  d3 ocn, rdn;
+  double r0, r1;
   bool hit;
 };
 __device__ __forceinline__ void safe_roots_d(d3 ro, f3 rd, f3 s, float4 q, SafeRootsD& D) {
-#pragma clang fp contract(off)
-  D.rd_rot = qrot_d(tod(rd), q);
-  const double nrm = sqrt(fmax(dotd(D.rd_rot, D.rd_rot), (double)kTiny));
-  const double rnrm = safe_rcp_d(nrm);
-  D.Trd = mkd(safe_mul_d(D.rd_rot.x, rnrm), safe_mul_d(D.rd_rot.y, rnrm), safe_mul_d(D.rd_rot.z, rnrm));
-  D.Tro = qrot_d(ro, q);
-  const double rsx = safe_rcp_d(s.x), rsy = safe_rcp_d(s.y), rsz = safe_rcp_d(s.z);
-  D.ocn = mkd(safe_mul_d(D.Tro.x, rsx), safe_mul_d(D.Tro.y, rsy), safe_mul_d(D.Tro.z, rsz));
-  D.rdn = mkd(safe_mul_d(D.Trd.x, rsx), safe_mul_d(D.Trd.y, rsy), safe_mul_d(D.Trd.z, rsz));
-  D.a = dotd(D.rdn, D.rdn);
-  D.bp = -dotd(D.ocn, D.rdn);
-  D.ra = safe_rcp_d(D.a);
-  D.ba = safe_mul_d(D.bp, D.ra);
-  D.l = D.ocn + D.ba * D.rdn;
-  D.omll = 1.0 - dotd(D.l, D.l);
-  D.h = D.a * D.omll;
-  D.c = dotd(D.ocn, D.ocn) - 1.0;
-  D.hit = !(D.h < 0.0);
-  D.sh = D.h < (double)kTiny ? 0.0 : sqrt(D.h);
-  D.sb = D.bp > 0.0 ? 1.0 : (D.bp < 0.0 ? -1.0 : 0.0);
-  D.qq = D.bp + D.sb * D.sh;
-  D.r0 = safe_div_d(D.c, D.qq);
-  D.r1 = safe_mul_d(D.qq, D.ra);
+  const d3 rd_rot = qrot_d(tod(rd), q);
+  const double rnrm = rsq_nr_d(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
+  const d3 Trd = mkd(safe_mul_d(rd_rot.x, rnrm), safe_mul_d(rd_rot.y, rnrm), safe_mul_d(rd_rot.z, rnrm));
+  const d3 Tro = qrot_d(ro, q);
+  const double rsx = safe_rcp_fast_d(s.x), rsy = safe_rcp_fast_d(s.y), rsz = safe_rcp_fast_d(s.z);
+  D.ocn = mkd(safe_mul_d(Tro.x, rsx), safe_mul_d(Tro.y, rsy), safe_mul_d(Tro.z, rsz));
+  D.rdn = mkd(safe_mul_d(Trd.x, rsx), safe_mul_d(Trd.y, rsy), safe_mul_d(Trd.z, rsz));
+  const double a = dotd(D.rdn, D.rdn);
+  const double bp = -dotd(D.ocn, D.rdn);
+  const double ra = safe_rcp_fast_d(a);
+  const double ba = safe_mul_d(bp, ra);
+  const d3 l = D.ocn + ba * D.rdn;
+  const double h = a * (1.0 - dotd(l, l));
+  const double c = dotd(D.ocn, D.ocn) - 1.0;
+  D.hit = !(h < 0.0);
+  const double sh = h < (double)kTiny ? 0.0 : h * rsq_nr_d(h);
+  const double qq = bp + (bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0)) * sh;
+  D.r0 = safe_mul_d(c, safe_rcp_fast_d(qq));
+  D.r1 = safe_mul_d(qq, ra);
 }
 // The ray of the integrator in FP64: unit direction (l2 normalization of the raw direction); the origin is
 // rayo + tmin * rayd evaluated in FP64 (shaders.slang:178-179).
@@ -241,7 +245,7 @@ __device__ __forceinline__ d3 ray_origin_fp64(f3 ro, f3 rdraw, float tmin) {
 // Roots of one primitive in FP64 from its raw parameters, as the forward integrates them: eliIntersect
 // (tri-intersect.slang:268-283) on the ray rotated by the raw quaternion (rotate_vector, safe-math.slang:237-244)
 // and divided by the scales. Returns whether 1 - |l|^2 >= 0 (the hit test of :275); the roots use it clamped at 0.
-// The backward evaluates the same function on the same inputs to unwind the forward's log-transmittance exactly.
+// Written without contraction, so that every kernel that inlines it (both tree walks) computes identical depths.
 __device__ __forceinline__ bool raw_roots_d(d3 o, d3 d, float4 q, f3 sc, f3 mu, double& t0, double& t1) {
 #pragma clang fp contract(off)
   const d3 Tro = qrot_d(o - tod(mu), q), Trd = qrot_d(d, q);
@@ -262,22 +266,8 @@ __device__ __forceinline__ bool raw_roots_d(d3 o, d3 d, float4 q, f3 sc, f3 mu, 
   return !(h < 0.0);
 }
 
-// The selected safe root in FP64, -1 on a miss (safe_intersect, tri-intersect.slang:443-456).
-__device__ __forceinline__ double safe_depth(d3 ro, f3 rd, f3 s, float4 q, int face) {
-  SafeRootsD D;
-  safe_roots_d(ro, rd, s, q, D);
-  if (!D.hit) return -1.0;
-  return face == 1 ? fmin(D.r0, D.r1) : fmax(D.r0, D.r1);
-}
-
-// FP64 forms of the safe-math adjoint helpers (safe-math.slang:68-106), for gradients only: reciprocals from the
-// hardware estimate (v_rcp_f64, v_rsq_f64) refined by two Newton steps, within a few ulp of the IEEE operations.
+// FP64 forms of the safe-math adjoint helpers (safe-math.slang:68-106), with the refined hardware reciprocals.
 __device__ __forceinline__ double clipd(double v) { return fmax(fmin(v, (double)kMaxVal), (double)kMinVal); }
-__device__ __forceinline__ double rcp_nr_d(double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  r = fma(fma(-b, r, 1.0), r, r);
-  return fma(fma(-b, r, 1.0), r, r);
-}
 // 1/b' and 1/b2' of a guarded denominator: b' = b, or kTiny where |b| < kTiny; b2' = b*b, or kTiny below it
 struct SafeRcpD {
   double r1, r2;
@@ -307,9 +297,7 @@ __device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro,
   // sqrt(h) and 1/sqrt(h) from the reciprocal square root estimate and two Newton steps
   double rs = 0.0, sh = 0.0;
   if (!(h < (double)kTiny)) {
-    rs = __builtin_amdgcn_rsq(h);
-    rs = rs * fma(-0.5 * h * rs, rs, 1.5);
-    rs = rs * fma(-0.5 * h * rs, rs, 1.5);
+    rs = rsq_nr_d(h);
     sh = h * rs;
   }
   const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
