@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal form")
     ap.add_argument("--no-inference", action="store_true", help="skip the inference-renderer frame timing")
     ap.add_argument("--mlp-epochs", type=int, default=12)
+    ap.add_argument("--hbm-probe-only", action="store_true",
+                    help="only the 1 GiB copy (a kernel of known traffic, for calibrating the PMC byte counters)")
     return ap.parse_args()
 
 
@@ -118,10 +120,32 @@ def cpu_baseline(sc, o, d, eye, sample, max_iters, nr_full):
     t_bwd = time.time() - t0
     per_ray = max(t_fwd, 0.0) / len(idx) + t_bwd / len(idx)
     total = t_sh + t_build + per_ray * nr_full
-    return dict(value=round(nr_full / total / 1e6, 6), unit="Mrays/s", cores=oracle.num_threads(), kind="port",
+    # the OpenMP threads that ran, bounded by the cores this process may use
+    cores = min(oracle.num_threads(), len(os.sched_getaffinity(0)))
+    return dict(value=round(nr_full / total / 1e6, 6), unit="Mrays/s", cores=cores, kind="port",
                 sample=(f"{len(idx)} of {nr_full} rays (every {stride}th pixel), FP32, CPU BVH build "
                         f"{t_build:.2f}s + SH prepass {t_sh:.2f}s counted once, per-ray fwd+bwd "
                         f"{1e3 * per_ray:.3f} ms extrapolated to the full image"))
+
+
+def hbm_copy_rate(dev, gib=1.0, reps=5):
+    """Box-measured HBM bandwidth: event-timed device-to-device copies of a 1 GiB buffer (larger than the L2 and
+    the 256 MB last-level cache together), read + write bytes per second."""
+    import torch
+    n = int(gib * 2**30) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"gbs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1), "bytes_per_copy": 2 * n * 4, "ms": round(ms, 4)}
 
 
 MLP_FLOPS_PER_RAY = 2 * (63 * 128 + 128 * 128 + 128 * 128 + 128)  # 81,920 (inference)
@@ -254,6 +278,9 @@ def main():
     from ever_amd.splinetracers import fast_ellipsoid_splinetracer as fes
 
     dev = torch.device("cuda:0")
+    if a.hbm_probe_only:
+        print(json.dumps({"hbm_copy": hbm_copy_rate(dev)}), flush=True)
+        return
     T.set_kbuf(a.kbuf)
     sc, o, d = scenes.config3(width=a.width, height=a.height, n=a.prims)
     eye = (0.0, 0.0, -4.0)
@@ -324,7 +351,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": fwd_kernel if dom == "forward" else "k_trace_backward",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "algorithmic_bytes": int(dom_bytes), "launch_ms": round(dom_ms, 4)},
+                     "algorithmic_bytes": int(dom_bytes), "launch_ms": round(dom_ms, 4),
+                     "peak_measured_copy": hbm_copy_rate(dev)},
     }
     if not a.no_inference:
         rec["inference"] = inference_frames(params, og, dg, eye, nr)

@@ -68,6 +68,32 @@ class ProposalNet:
             C.check(L.ever_mlp_sync_bf16(C.ptr(self.params), C.ptr(self.params_bf16), C.stream_ptr()),
                     "ever_mlp_sync_bf16")
 
+    def save(self, path):
+        """Write the FP32 parameters, the Adam moments and the step count to a safetensors file (a checkpoint from
+        which training resumes exactly)."""
+        from safetensors.torch import save_file
+        off, count = layout()
+        save_file({"params": self.params.detach().cpu().contiguous(), "adam_m": self.adam_m.cpu().contiguous(),
+                   "adam_v": self.adam_v.cpu().contiguous(),
+                   "step": torch.tensor([self.step], dtype=torch.int64)},
+                  path, metadata={"format": "ever_amd.ProposalNet", "hidden": str(HID), "inputs": str(IN),
+                                  "param_count": str(count)})
+
+    @classmethod
+    def load(cls, path, device=None):
+        """A network with the parameters, Adam state and step count of a file written by save()."""
+        from safetensors import safe_open
+        net = cls(device)
+        with safe_open(path, framework="pt") as f:
+            meta = f.metadata() or {}
+            if meta.get("format") != "ever_amd.ProposalNet" or int(meta.get("param_count", -1)) != net.params.numel():
+                raise ValueError(f"{path} is not a proposal network checkpoint of this layout")
+            net.set_params(f.get_tensor("params").to(net.device))
+            net.adam_m.copy_(f.get_tensor("adam_m"))
+            net.adam_v.copy_(f.get_tensor("adam_v"))
+            net.step = int(f.get_tensor("step")[0])
+        return net
+
     def _workspace(self, nr):
         need = C.lib().ever_mlp_workspace_bytes(nr)
         if self._ws is None or self._ws.numel() < need:

@@ -112,6 +112,28 @@ def test_training_on_gpu_lowers_the_loss():
     assert last < 0.25 * first
 
 
+def test_checkpoint_resumes_training_exactly(tmp_path):
+    # parameters, Adam moments and step count round-trip through a safetensors file: the reloaded network infers
+    # the same depths and its next training step equals the original's
+    net = PR.ProposalNet(D, seed=6)
+    o, d = _rays(65536, 6)
+    o, d = o.to(D).contiguous(), d.to(D).contiguous()
+    target = (2.0 + o.norm(dim=1)).contiguous()
+    for _ in range(3):
+        net.train_step(o, d, target, 0.0, 0.5, 2e-3)
+    path = str(tmp_path / "proposal.safetensors")
+    net.save(path)
+    net2 = PR.ProposalNet.load(path, D)
+    assert net2.step == net.step
+    assert torch.equal(net2.params, net.params) and torch.equal(net2.adam_m, net.adam_m)
+    assert torch.equal(net2.adam_v, net.adam_v)
+    assert torch.equal(net2.log_depth(o, d), net.log_depth(o, d))
+    l1 = float(net.train_step(o, d, target, 0.0, 0.5, 2e-3))
+    l2 = float(net2.train_step(o, d, target, 0.0, 0.5, 2e-3))
+    # (the batch loss is a float atomic sum: its last bits depend on the order of the additions)
+    assert abs(l1 - l2) <= 1e-6 * abs(l1) and torch.equal(net2.params, net.params)
+
+
 @pytest.mark.parametrize("traversal", ["ray", "wave"])
 def test_proposal_traces_equal_plain(traversal):
     print(f"case: proposal vs plain, {traversal}", flush=True)
