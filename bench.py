@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--prims", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--kbuf", type=int, default=16)
+    ap.add_argument("--kbuf", type=int, default=19)
     ap.add_argument("--max-iters", type=int, default=500)
     ap.add_argument("--cpu-sample", type=int, default=131072, help="rays traced by the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -131,7 +131,8 @@ def test_checkpoint_resumes_training_exactly(tmp_path):
     l1 = float(net.train_step(o, d, target, 0.0, 0.5, 2e-3))
     l2 = float(net2.train_step(o, d, target, 0.0, 0.5, 2e-3))
     # (the batch loss is a float atomic sum: its last bits depend on the order of the additions)
-    assert abs(l1 - l2) <= 1e-6 * abs(l1) and torch.equal(net2.params, net.params)
+    assert abs(l1 - l2) <= 1e-5 * abs(l1)
+    assert torch.equal(net2.params, net.params)
 
 
 @pytest.mark.parametrize("traversal", ["ray", "wave"])
