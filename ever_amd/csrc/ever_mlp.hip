@@ -16,8 +16,18 @@
 
 namespace ever {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// gfx950's v_mfma_f32_16x16x32_bf16: D(16x16) += A(16x32) B(32x16), FP32 accumulation. Lane l supplies row l % 16 of
+// A and column l % 16 of B, each with the eight k of its lane group (k = 8 * (l / 16) + 0..7 here: the product pairs
+// A's and B's elements by lane group and slot, so any k order both operands share gives the same sum), and receives
+// D[4 * (l / 16) + q][l % 16], q = 0..3.
+__device__ __forceinline__ f32x4 mfma16x16x32(const uint16_t* a, const uint16_t* b, f32x4 c) {
+  const uint4 av = *reinterpret_cast<const uint4*>(a), bv = *reinterpret_cast<const uint4*>(b);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), c,
+                                                 0, 0, 0);
+}
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -29,10 +39,11 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 // ---------------------------------------------------------------------------------------------------------------
 // C[m][n] = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bf16), FP32 accumulation on
-// v_mfma_f32_16x16x16_bf16. Block tile 64 x 64 (4 waves, 32 x 32 each = 2 x 2 MFMA tiles), K in steps of 16.
-// blockIdx.z selects a K chunk [z*kc, min(K,(z+1)*kc)); with more than one chunk the block writes FP32 partials
-// part[z][m][n], else the epilogue: + bias[n], ReLU, FP32 (c32 with strides) and/or bf16 (c16 with strides).
-// Requires M % 64 == 0, N % 64 == 0, kc % 16 == 0, K % 16 == 0 (checked by the host launcher).
+// v_mfma_f32_16x16x32_bf16. Block tile 64 x 64 (4 waves, 32 x 32 each = 2 x 2 MFMA tiles), K in steps of 32 (a chunk's
+// last step zero-padded past its end). blockIdx.z selects a K chunk [z*kc, min(K,(z+1)*kc)); with more than one chunk
+// the block writes FP32 partials part[z][m][n], else the epilogue: + bias[n], ReLU, FP32 (c32 with strides) and/or
+// bf16 (c16 with strides). Requires M % 64 == 0, N % 64 == 0, kc % 16 == 0, K % 16 == 0 (checked by the host
+// launcher).
 struct GemmArgs {
   const uint16_t* A;
   int64_t sam, sak;
@@ -49,8 +60,8 @@ struct GemmArgs {
 };
 
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
-  __shared__ uint16_t As[64][16 + 4];  // [m][k], padded rows
-  __shared__ uint16_t Bs[64][16 + 4];  // [n][k]
+  __shared__ __align__(16) uint16_t As[64][32 + 8];  // [m][k], padded rows (16-byte aligned)
+  __shared__ __align__(16) uint16_t Bs[64][32 + 8];  // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * 64, n0 = (int64_t)blockIdx.y * 64;
   const int64_t kb = (int64_t)blockIdx.z * g.kc, ke = min(g.K, kb + g.kc);
@@ -60,28 +71,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t k0 = kb; k0 < ke; k0 += 16) {
-    // 64 x 16 elements of A and of B: 4 each per thread
+  for (int64_t k0 = kb; k0 < ke; k0 += 32) {
+    // 64 x 32 elements of A and of B: 8 each per thread (zero past the chunk's end)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + 256 * e, r = idx >> 4, c = idx & 15;
-      As[r][c] = g.A[(m0 + r) * g.sam + (k0 + c) * g.sak];
-      Bs[r][c] = g.B[(k0 + c) * g.sbk + (n0 + r) * g.sbn];
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + 256 * e, r = idx >> 5, c = idx & 31;
+      const bool in = k0 + c < ke;
+      As[r][c] = in ? g.A[(m0 + r) * g.sam + (k0 + c) * g.sak] : (uint16_t)0;
+      Bs[r][c] = in ? g.B[(k0 + c) * g.sbk + (n0 + r) * g.sbn] : (uint16_t)0;
     }
     __syncthreads();
-    const int row = lane & 15, kq = (lane >> 4) * 4;
-    bf16x4 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const uint16_t* pa = &As[wm + 16 * i + row][kq];
-      const uint16_t* pb = &Bs[wn + 16 * i + row][kq];
-      a[i] = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-      b[i] = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
-    }
+    const int row = lane & 15, k8 = (lane >> 4) * 8;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x16x32(&As[wm + 16 * i + row][k8], &Bs[wn + 16 * j + row][k8], acc[i][j]);
     __syncthreads();
   }
   // lane holds D[4*(lane/16) + q][lane % 16] of each 16 x 16 tile
@@ -156,6 +160,42 @@ __device__ __forceinline__ void encode_ray(f3 o, f3 d, float* x) {
       x[c++] = cs;
     }
   x[c] = 0.f;  // c == 63
+}
+
+// The features of encode_ray at the same indices, split over four lanes per ray by `part` (0..3): part 0 the plain
+// features (o, p, d) and sin/cos of p at frequencies 1, 2; part 1 p at 4, 8; part 2 p at 16 and d at 1; part 3 d at
+// 2, 4, 8 (nine sin/cos pairs at most per lane) and the zero column. Written as bf16 into row x.
+__device__ __forceinline__ void encode_ray_part(f3 o, f3 d, int part, uint16_t* x) {
+  const float pd = dot3(o, d);
+  const f3 p = o - pd * d;
+  const float pv[3] = {0.25f * p.x, 0.25f * p.y, 0.25f * p.z};
+  const float dv[3] = {d.x, d.y, d.z};
+  auto pair = [&](const float* v, int base, int j) {  // sin/cos of 2^j pi v[a] at base + 2 * (3 j + a) + 0/1
+    for (int a = 0; a < 3; ++a) {
+      float sn, cs;
+      sincospif(ldexpf(v[a], j), &sn, &cs);
+      x[base + 2 * (3 * j + a)] = f2bf(sn);
+      x[base + 2 * (3 * j + a) + 1] = f2bf(cs);
+    }
+  };
+  if (part == 0) {
+    x[0] = f2bf(0.25f * o.x); x[1] = f2bf(0.25f * o.y); x[2] = f2bf(0.25f * o.z);
+    for (int a = 0; a < 3; ++a) x[3 + a] = f2bf(pv[a]);
+    for (int a = 0; a < 3; ++a) x[36 + a] = f2bf(dv[a]);
+    pair(pv, 6, 0);
+    pair(pv, 6, 1);
+  } else if (part == 1) {
+    pair(pv, 6, 2);
+    pair(pv, 6, 3);
+  } else if (part == 2) {
+    pair(pv, 6, 4);
+    pair(dv, 39, 0);
+  } else {
+    pair(dv, 39, 1);
+    pair(dv, 39, 2);
+    pair(dv, 39, 3);
+    x[63] = 0;
+  }
 }
 
 __global__ void k_encode(const float* __restrict__ rayo, const float* __restrict__ rayd, int64_t nr, int64_t mpad,
@@ -360,8 +400,9 @@ static hipError_t mlp_hidden(const float* p, const uint16_t* pb, char* ws, const
 // time; activations stay in LDS. Same arithmetic as the layer-by-layer path (bf16 operands, FP32 accumulation,
 // bias + ReLU, bf16 activations, FP32 head).
 constexpr int kFuseWaves = 8;
-constexpr int kPadIn = kMlpIn + 4, kPadHid = kMlpHid + 4;  // row pitches in bf16 elements (bank spread)
-struct FusedLds {
+// row pitches in bf16 elements: multiples of 8 (16-byte operand loads), offset by 8 from the row length (bank spread)
+constexpr int kPadIn = kMlpIn + 8, kPadHid = kMlpHid + 8;
+struct __align__(16) FusedLds {
   uint16_t w1[kMlpHid][kPadIn];
   uint16_t w2[kMlpHid][kPadHid];
   uint16_t w3[kMlpHid][kPadHid];
@@ -377,17 +418,11 @@ __device__ __forceinline__ void fused_layer(const uint16_t (*W)[PITCH], const fl
   f32x4 acc[8];
 #pragma unroll
   for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int row = lane & 15, kq = (lane >> 4) * 4;
+  const int row = lane & 15, k8 = (lane >> 4) * 8;
 #pragma unroll
-  for (int k0 = 0; k0 < KIN; k0 += 16) {
-    const uint16_t* pa = &act[row][k0 + kq];
-    const bf16x4 a = {(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
+  for (int k0 = 0; k0 < KIN; k0 += 32) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const uint16_t* pb = &W[t * 16 + row][k0 + kq];
-      const bf16x4 b = {(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-    }
+    for (int t = 0; t < 8; ++t) acc[t] = mfma16x16x32(&act[row][k0 + k8], &W[t * 16 + row][k0 + k8], acc[t]);
   }
   __builtin_amdgcn_wave_barrier();  // every lane has read its operands before any lane overwrites act
   const int col = lane & 15, rq = (lane >> 4) * 4;
@@ -425,19 +460,31 @@ __global__ __launch_bounds__(64 * kFuseWaves) void k_mlp_fused_infer(const float
   __syncthreads();
   uint16_t (*act)[kPadHid] = S.act[wave];
   const int64_t tiles = (nr + 15) / 16;
-  for (int64_t tile = (int64_t)blockIdx.x * kFuseWaves + wave; tile < tiles; tile += (int64_t)gridDim.x * kFuseWaves) {
-    // encoding: lanes 0..15 one ray each
-    if (lane < 16) {
-      const int64_t r = tile * 16 + lane;
-      float x[kMlpIn];
+  const int64_t stride = (int64_t)gridDim.x * kFuseWaves;
+  // the ray of this lane in a tile (lane % 16), loaded one tile ahead so that the loads overlap the previous tile
+  auto load_ray = [&](int64_t tile, f3& o, f3& d) {
+    const int64_t r = tile * 16 + (lane & 15);
+    if (tile < tiles && r < nr) {
+      o = ld3(rayo + 3 * r);
+      d = ld3(rayd + 3 * r);
+    } else {
+      o = mk3(0.f, 0.f, 0.f);
+      d = mk3(0.f, 0.f, 0.f);
+    }
+  };
+  f3 o_next, d_next;
+  load_ray((int64_t)blockIdx.x * kFuseWaves + wave, o_next, d_next);
+  for (int64_t tile = (int64_t)blockIdx.x * kFuseWaves + wave; tile < tiles; tile += stride) {
+    const f3 ro = o_next, rd = d_next;
+    load_ray(tile + stride, o_next, d_next);
+    // encoding: four lanes per ray (lane % 16 the ray, lane / 16 a quarter of the features)
+    {
+      const int64_t r = tile * 16 + (lane & 15);
       if (r < nr) {
-        const f3 rd = ld3(rayd + 3 * r);
-        encode_ray(ld3(rayo + 3 * r) + tmin * rd, l2n(rd), x);
+        encode_ray_part(ro + tmin * rd, l2n(rd), lane >> 4, act[lane & 15]);
       } else {
-        for (int i = 0; i < kMlpIn; ++i) x[i] = 0.f;
+        for (int i = lane >> 4; i < kMlpIn; i += 4) act[lane & 15][i] = 0;
       }
-#pragma unroll
-      for (int i = 0; i < kMlpIn; ++i) act[lane][i] = f2bf(x[i]);
     }
     __builtin_amdgcn_wave_barrier();
     fused_layer<kMlpIn, kPadIn>(S.w1, S.bias[0], act, lane);
@@ -446,8 +493,20 @@ __global__ __launch_bounds__(64 * kFuseWaves) void k_mlp_fused_infer(const float
     // head: lane -> row lane % 16, quarter lane / 16 of the 128 inputs; sum the quarters across the wave
     const int row = lane & 15, qtr = lane >> 4;
     float s = 0.f;
+    const uint4* hv = reinterpret_cast<const uint4*>(&act[row][qtr * 32]);
+    const float4* wv = reinterpret_cast<const float4*>(&S.w4[qtr * 32]);
 #pragma unroll
-    for (int k = 0; k < 32; ++k) s = fmaf(bf2f(act[row][qtr * 32 + k]), S.w4[qtr * 32 + k], s);
+    for (int c = 0; c < 4; ++c) {
+      const uint4 h = hv[c];
+      const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 w = wv[2 * c + j / 2];
+        const float w0 = (j & 1) ? w.z : w.x, w1 = (j & 1) ? w.w : w.y;
+        s = fmaf(__uint_as_float(hw[j] << 16), w0, s);
+        s = fmaf(__uint_as_float(hw[j] & 0xffff0000u), w1, s);
+      }
+    }
     s += __shfl_xor(s, 16);
     s += __shfl_xor(s, 32);
     const int64_t r = tile * 16 + row;
