@@ -147,33 +147,25 @@ __device__ __forceinline__ void event_load(const BwdArgs& A, int32_t id, int32_t
   R.scl = ld3(A.scales + 3 * p);
 }
 
-// One event's primitive and its selected safe root: the FP64 depth, and the ray in the primitive's normalized
-// frame that the adjoint of the root starts from. The forward accumulated its log-transmittance for this kernel
-// with the same depths (safe_roots_d), so the states unwind exactly. The loop below needs them twice, as the depth
-// of the previous event and then as the event whose adjoint is taken; they are computed once and carried over.
+// One event's primitive and its selected safe root: the FP64 depth and the root's gradient w.r.t. the ray origin
+// relative to the mean, the raw direction, the scale and the raw quaternion (safe_roots_jac_d; the chain from the
+// root's adjoint tb to these four is linear in tb, so an event's contribution is tb times it: the adjoint's clamps at
+// +-1e20 bind only where such a product exceeds them). The forward accumulated its log-transmittance for this kernel
+// with the same depths, so the states unwind exactly. The loop below needs them twice,
+// as the depth of the previous event and then as the event whose adjoint is taken; they are computed once and
+// carried over.
 struct EventGeo {
-  d3 ocn, rdn;  // rotated origin and unit direction divided by the scales (FP64)
-  float4 q, cr;
-  f3 mean, scl, ro;  // ro = (float)(origin - mean)
-  bool ok;           // the ray meets the ellipsoid
-  bool sel_r0;       // the selected root is r0 = c / q
+  float4 cr;
+  f3 mean;
+  f3 jro, jrd, js;  // d root / d(origin - mean), d root / d raw direction, d root / d scale
+  float4 jq;        // d root / d raw quaternion
+  bool ok;          // the ray meets the ellipsoid
 };
 __device__ __forceinline__ double event_geo(d3 org, f3 dir, int32_t id, const EventRaw& R, EventGeo& G) {
-  G.q = R.q;
   G.cr = R.cr;
   G.mean = R.mean;
-  G.scl = R.scl;
-  const d3 rod = org - tod(G.mean);
-  G.ro = tof(rod);
-  SafeRootsD D;
-  safe_roots_d(rod, dir, G.scl, G.q, D);
-  G.ocn = D.ocn;
-  G.rdn = D.rdn;
-  G.ok = D.hit;
   // entry (face 1): the smaller root, exit: the larger (safe_intersect, tri-intersect.slang:443-456)
-  G.sel_r0 = (id & 1) ? (D.r0 < D.r1) : (D.r0 > D.r1);
-  if (!D.hit) return -1.0;
-  return (id & 1) ? fmin(D.r0, D.r1) : fmax(D.r0, D.r1);
+  return safe_roots_jac_d(org - tod(G.mean), dir, R.scl, R.q, (id & 1) != 0, G.jro, G.jrd, G.js, G.jq, G.ok);
 }
 
 __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_t col, uint32_t* hkeys,
@@ -313,7 +305,12 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     const f3 mean = G.mean;
     f3 rob = mk3(0.f, 0.f, 0.f), rdb = mk3(0.f, 0.f, 0.f), sbar = mk3(0.f, 0.f, 0.f);
     float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (G.ok) safe_roots_bwd_d(G.ocn, G.rdn, G.sel_r0, G.ro, dir, G.scl, G.q, tb, rob, rdb, sbar, qb);
+    if (G.ok) {
+      rob = tb * G.jro;
+      rdb = tb * G.jrd;
+      sbar = tb * G.js;
+      qb = make_float4(tb * G.jq.x, tb * G.jq.y, tb * G.jq.z, tb * G.jq.w);
+    }
     orgb = orgb + rob;
     dirb = dirb + rdb;
     {

@@ -200,36 +200,6 @@ __device__ __forceinline__ double rsq_nr_d(double x) {
 // 1 / b' of safe_div's guarded denominator (b' = b, or kTiny where |b| < kTiny)
 __device__ __forceinline__ double safe_rcp_fast_d(double b) { return rcp_nr_d(fabs(b) < (double)kTiny ? (double)kTiny : b); }
 
-// Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344): the rotated raw direction is
-// l2-normalized, divisions use safe_div. Evaluated in FP64, so that 1 - |l|^2 and the root depths keep their
-// precision at grazing incidence and far from the ray origin. The adjoint of the selected root
-// (safe_roots_bwd_d) starts from ocn and rdn.
-struct SafeRootsD {
-  d3 ocn, rdn;
-  double r0, r1;
-  bool hit;
-};
-__device__ __forceinline__ void safe_roots_d(d3 ro, f3 rd, f3 s, float4 q, SafeRootsD& D) {
-  const d3 rd_rot = qrot_d(tod(rd), q);
-  const double rnrm = rsq_nr_d(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
-  const d3 Trd = mkd(safe_mul_d(rd_rot.x, rnrm), safe_mul_d(rd_rot.y, rnrm), safe_mul_d(rd_rot.z, rnrm));
-  const d3 Tro = qrot_d(ro, q);
-  const double rsx = safe_rcp_fast_d(s.x), rsy = safe_rcp_fast_d(s.y), rsz = safe_rcp_fast_d(s.z);
-  D.ocn = mkd(safe_mul_d(Tro.x, rsx), safe_mul_d(Tro.y, rsy), safe_mul_d(Tro.z, rsz));
-  D.rdn = mkd(safe_mul_d(Trd.x, rsx), safe_mul_d(Trd.y, rsy), safe_mul_d(Trd.z, rsz));
-  const double a = dotd(D.rdn, D.rdn);
-  const double bp = -dotd(D.ocn, D.rdn);
-  const double ra = safe_rcp_fast_d(a);
-  const double ba = safe_mul_d(bp, ra);
-  const d3 l = D.ocn + ba * D.rdn;
-  const double h = a * (1.0 - dotd(l, l));
-  const double c = dotd(D.ocn, D.ocn) - 1.0;
-  D.hit = !(h < 0.0);
-  const double sh = h < (double)kTiny ? 0.0 : h * rsq_nr_d(h);
-  const double qq = bp + (bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0)) * sh;
-  D.r0 = safe_mul_d(c, safe_rcp_fast_d(qq));
-  D.r1 = safe_mul_d(qq, ra);
-}
 // The ray of the integrator in FP64: unit direction (l2 normalization of the raw direction); the origin is
 // rayo + tmin * rayd evaluated in FP64 (shaders.slang:178-179).
 __device__ __forceinline__ d3 ray_dir_fp64(f3 rdraw) {
@@ -277,15 +247,26 @@ __device__ __forceinline__ SafeRcpD safe_rcp_fd(double b) {
   return {r1, b * b < (double)kTiny ? 1.0 / (double)kTiny : r1 * r1};
 }
 
-// Adjoint of the selected safe root (tri-intersect.slang:286-344) w.r.t. ro = origin - mean, rd = raw direction,
-// scale and raw quaternion, in FP64 from the ray in the primitive's normalized frame (ocn = rotated origin / scale,
-// rdn = rotated unit direction / scale, both from safe_roots_d). Along a thin axis ocn and t * rdn are large and
-// nearly opposite (their sum is the hit point on the unit sphere), so the scale gradient -(ocn*ocnb + rdn*rdnb)/s
-// cancels by |ocn|: in FP32 that loses three to four digits, in FP64 none that matter. sel_r0: the selected root
-// is r0 = c / q (else r1 = q / a).
-__device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro, f3 rd, f3 s, float4 q, float tb,
-                                                 f3& rob, f3& rdb, f3& sbar, float4& qb) {
-  // the forward quantities, recomputed as safe_roots_d computes them
+// Safe ellipsoid roots with the backward's conventions (tri-intersect.slang:286-344: the rotated raw direction is
+// l2-normalized, divisions use safe_div) and, when the ray meets the ellipsoid, the gradient of the root the face
+// selects (entry: the smaller, safe_intersect, tri-intersect.slang:443-456) w.r.t. rod = origin - mean, rd = raw
+// direction, the scale and the raw quaternion: the adjoint of the root with a unit upstream gradient. Evaluated in
+// FP64 (reciprocals and reciprocal square roots from the hardware estimates refined by Newton steps) in the
+// primitive's normalized frame (o = rotated origin / scale, d = rotated unit direction / scale), so that 1 - |l|^2
+// and the depths keep their precision at grazing incidence and far from the ray origin, and so that the scale
+// gradient -(o * ob + d * db) / s, which cancels by |o| along a thin axis (o and t * d are large and nearly opposite
+// there, their sum being the hit point on the unit sphere), loses no digits that matter. Returns the selected root
+// (-1 for a miss); hit: 1 - |l|^2 >= 0.
+__device__ __forceinline__ double safe_roots_jac_d(d3 rod, f3 rd, f3 s, float4 q, bool entry, f3& jro, f3& jrd,
+                                                   f3& js, float4& jq, bool& hit) {
+  // the roots
+  const d3 rd_rot = qrot_d(tod(rd), q);
+  const double rnrm = rsq_nr_d(fmax(dotd(rd_rot, rd_rot), (double)kTiny));
+  const d3 Trd = mkd(safe_mul_d(rd_rot.x, rnrm), safe_mul_d(rd_rot.y, rnrm), safe_mul_d(rd_rot.z, rnrm));
+  const d3 Tro = qrot_d(rod, q);
+  const SafeRcpD rx = safe_rcp_fd(s.x), ry = safe_rcp_fd(s.y), rz = safe_rcp_fd(s.z);
+  const d3 o = mkd(safe_mul_d(Tro.x, rx.r1), safe_mul_d(Tro.y, ry.r1), safe_mul_d(Tro.z, rz.r1));
+  const d3 d = mkd(safe_mul_d(Trd.x, rx.r1), safe_mul_d(Trd.y, ry.r1), safe_mul_d(Trd.z, rz.r1));
   const double a = dotd(d, d);
   const double bp = -dotd(o, d);
   const SafeRcpD ra = safe_rcp_fd(a);
@@ -294,7 +275,7 @@ __device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro,
   const double omll = 1.0 - dotd(l, l);
   const double h = a * omll;
   const double c = dotd(o, o) - 1.0;
-  // sqrt(h) and 1/sqrt(h) from the reciprocal square root estimate and two Newton steps
+  hit = !(h < 0.0);
   double rs = 0.0, sh = 0.0;
   if (!(h < (double)kTiny)) {
     rs = rsq_nr_d(h);
@@ -302,9 +283,14 @@ __device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro,
   }
   const double sb = bp > 0.0 ? 1.0 : (bp < 0.0 ? -1.0 : 0.0);
   const double qq = bp + sb * sh;
-  const double r0b = sel_r0 ? (double)tb : 0.0, r1b = sel_r0 ? 0.0 : (double)tb;
-  // r0 = safe_div(c, qq), r1 = safe_div(qq, a)
   const SafeRcpD rq = safe_rcp_fd(qq);
+  const double r0 = safe_mul_d(c, rq.r1), r1 = safe_mul_d(qq, ra.r1);
+  jro = jrd = js = mk3(0.f, 0.f, 0.f);
+  jq = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!hit) return -1.0;
+  const bool sel_r0 = entry ? (r0 < r1) : (r0 > r1);
+  // the adjoint of the selected root with a unit upstream gradient
+  const double r0b = sel_r0 ? 1.0 : 0.0, r1b = sel_r0 ? 0.0 : 1.0;
   const double cb = clipd(r0b * rq.r1);
   double qqb = clipd(-c * rq.r2 * r0b) + clipd(r1b * ra.r1);
   double ab = clipd(-qq * ra.r2 * r1b);
@@ -314,22 +300,20 @@ __device__ __forceinline__ void safe_roots_bwd_d(d3 o, d3 d, bool sel_r0, f3 ro,
   d3 ocnb = (2.0 * cb) * o + lb;
   const double bab = dotd(lb, d);
   d3 rdnb = ba * lb;
-  // ba = safe_div(bp, a)
   const double bpb = qqb + clipd(bab * ra.r1);
   ab += clipd(-bp * ra.r2 * bab);
   ocnb = ocnb + (-bpb) * d;
   rdnb = rdnb + (-bpb) * o + (2.0 * ab) * d;
-  // ocn = safe_div(Tro, s), rdn = safe_div(Trd, s), with Tro = ocn * s and Trd = rdn * s
-  const SafeRcpD rx = safe_rcp_fd(s.x), ry = safe_rcp_fd(s.y), rz = safe_rcp_fd(s.z);
   const f3 Trob = mk3((float)clipd(ocnb.x * rx.r1), (float)clipd(ocnb.y * ry.r1), (float)clipd(ocnb.z * rz.r1));
   const f3 Trdb = mk3((float)clipd(rdnb.x * rx.r1), (float)clipd(rdnb.y * ry.r1), (float)clipd(rdnb.z * rz.r1));
   const double sx = (double)s.x, sy = (double)s.y, sz = (double)s.z;
-  sbar.x += (float)(clipd(-(o.x * sx) * rx.r2 * ocnb.x) + clipd(-(d.x * sx) * rx.r2 * rdnb.x));
-  sbar.y += (float)(clipd(-(o.y * sy) * ry.r2 * ocnb.y) + clipd(-(d.y * sy) * ry.r2 * rdnb.y));
-  sbar.z += (float)(clipd(-(o.z * sz) * rz.r2 * ocnb.z) + clipd(-(d.z * sz) * rz.r2 * rdnb.z));
-  qrot_bwd(ro, q, Trob, rob, qb);
+  js.x = (float)(clipd(-(o.x * sx) * rx.r2 * ocnb.x) + clipd(-(d.x * sx) * rx.r2 * rdnb.x));
+  js.y = (float)(clipd(-(o.y * sy) * ry.r2 * ocnb.y) + clipd(-(d.y * sy) * ry.r2 * rdnb.y));
+  js.z = (float)(clipd(-(o.z * sz) * rz.r2 * ocnb.z) + clipd(-(d.z * sz) * rz.r2 * rdnb.z));
+  qrot_bwd(tof(rod), q, Trob, jro, jq);
   const f3 rrb = l2n_bwd(qrot(rd, q), Trdb);
-  qrot_bwd(rd, q, rrb, rdb, qb);
+  qrot_bwd(rd, q, rrb, jrd, jq);
+  return entry ? fmin(r0, r1) : fmax(r0, r1);
 }
 
 // Relaxed device-scope float atomic add (global_atomic_add_f32 on gfx950; result unused).

@@ -279,7 +279,7 @@ __device__ __forceinline__ RayGeo ray_geo(const FwdArgs& A, int64_t r) {
 // (C0, C1, C2, csx) (csy, dpx, dpy, iter).
 struct IntState {
   double D0, D1, D2, D3, t_state, logT;
-  // the same accumulation with the backward's safe roots (rotated direction l2-normalized, safe_roots_d): the
+  // the same accumulation with the backward's safe roots (rotated direction l2-normalized, safe_roots_jac_d): the
   // log-transmittance the adjoint starts from and unwinds event by event with exactly these depths
   double ts_state, logT_s;
   double tref64;  // FP64 depth of the last event of the last completed 16-event round
