@@ -15,8 +15,9 @@
 namespace ever {
 
 constexpr int kBwdBlock = 256;
-// per-block LDS table of the primitives the block's rays touch (a 256-ray screen tile touches about 50 at 1M/1080p)
-constexpr int kHashSlots = 512;
+// per-block LDS table of the primitives the block's rays touch (a 256-ray screen tile touches about 50 at 1M/1080p);
+// 256 slots of 14 FP64 fields keep the block's LDS at 30 KB, so that three blocks share a CU
+constexpr int kHashSlots = 256;
 constexpr int kAccFields = 14;    // mean 3, scale 3, quat 4, density 1, colour 3
 constexpr uint32_t kEmptySlot = 0xffffffffu;
 
@@ -75,7 +76,7 @@ __device__ __forceinline__ void scatter_prim(const BwdArgs& A, uint32_t* hkeys, 
                                              const float (&v)[kAccFields]) {
   // find the slot first (the lanes of a wave take different numbers of probes), then add with all lanes together:
   // one copy of the 14 atomics, issued once per flush by every flushing lane of the wave
-  uint32_t h = (p * 2654435761u) >> 23;  // 9 bits
+  uint32_t h = (p * 2654435761u) >> 24;  // 8 bits: kHashSlots = 256
   bool found = false;
 #pragma unroll 1
   for (int probe = 0; probe < 16; ++probe) {
@@ -207,18 +208,12 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   }
 
   const int32_t view = A.vcol ? A.view_index[r] : 0;
-  // software pipeline over the events k = E-1 .. 0: event ids are read two events ahead, primitive data one
-  // event ahead, so the loads of iteration k are issued during iteration k + 1
+  // software pipeline over the events k = E-1 .. 0: event ids are read three events ahead; the primitive data of
+  // event k - 1, which iteration k needs first (its depth), is loaded during iteration k + 1
   const int32_t* tcol = A.tri + col;
   int32_t id = tcol[(int64_t)(E - 1) * A.tri_stride];
   int32_t id_next = E >= 2 ? tcol[(int64_t)(E - 2) * A.tri_stride] : 0;   // event k - 1
   int32_t id_next2 = E >= 3 ? tcol[(int64_t)(E - 3) * A.tri_stride] : 0;  // event k - 2
-  // contributions of consecutive events of one primitive (its exit and entry are often neighbours) are summed
-  // in registers and reach the block's table once: its float atomics in LDS are the adjoint's dominant cost
-  uint32_t pend_key = kEmptySlot;
-  float pend[kAccFields];
-#pragma unroll
-  for (int j = 0; j < kAccFields; ++j) pend[j] = 0.f;
   EventGeo G;
   double tk;
   {
@@ -229,9 +224,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   EventRaw Rn;  // raw data of event k - 1
   if (E >= 2) event_load(A, id_next, view, Rn);
   for (int k = E - 1; k >= 0; --k) {
-    EventRaw Rn2;  // raw data of event k - 2 and the id of event k - 3, for the next iterations
     int32_t id_next3 = 0;
-    if (k >= 2) event_load(A, id_next2, view, Rn2);
     if (k >= 3) id_next3 = tcol[(int64_t)(k - 3) * A.tri_stride];
     int32_t id_prev = 0;
     double tprev = 0.0;
@@ -240,6 +233,8 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
       id_prev = id_next;
       tprev = event_geo(org, dir, id_prev, Rn, Gp);
     }
+    // raw data of event k - 2 into the registers event k - 1's just left, for the next iteration
+    if (k >= 2) event_load(A, id_next2, view, Rn);
     const int64_t p = id >> 1;
     const int face = id & 1;
     const float4 cr = G.cr;
@@ -317,15 +312,7 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
       const float v[kAccFields] = {-rob.x, -rob.y, -rob.z, sbar.x, sbar.y, sbar.z, qb.x, qb.y, qb.z, qb.w,
                                    densb, colb0, colb1, colb2};
       const uint32_t key = A.vcol ? (uint32_t)(view * A.n + p) : (uint32_t)p;
-      if (key == pend_key) {
-#pragma unroll
-        for (int j = 0; j < kAccFields; ++j) pend[j] += v[j];
-      } else {
-        if (pend_key != kEmptySlot) scatter_prim(A, hkeys, hvals, pend_key, pend);
-        pend_key = key;
-#pragma unroll
-        for (int j = 0; j < kAccFields; ++j) pend[j] = v[j];
-      }
+      scatter_prim(A, hkeys, hvals, key, v);
     }
     float* ac = A.acc + 16 * p;
     if (inv) {
@@ -347,11 +334,9 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
     id = id_prev;
     tk = tprev;
     G = Gp;
-    Rn = Rn2;
     id_next = id_next2;
     id_next2 = id_next3;
   }
-  if (pend_key != kEmptySlot) scatter_prim(A, hkeys, hvals, pend_key, pend);
   *dinit = make_float4(Db0, Db1, Db2, Db3);
   float* o = A.drayo + 3 * r;
   float* dd = A.drayd + 3 * r;
@@ -363,7 +348,8 @@ __device__ __forceinline__ void backward_ray(const BwdArgs& A, int64_t r, int64_
   dd[2] = clipf(dirb.z, -1e3f, 1e3f);
 }
 
-__global__ __launch_bounds__(kBwdBlock) void k_trace_backward(BwdArgs A) {
+// Compiled for three waves per SIMD (168 VGPRs, a few spilled): measured faster than two without spills.
+__global__ __launch_bounds__(kBwdBlock, 3) void k_trace_backward(BwdArgs A) {
   __shared__ uint32_t hkeys[kHashSlots];
   __shared__ double hvals[kAccFields][kHashSlots];
   for (int i = threadIdx.x; i < kHashSlots; i += kBwdBlock) {
