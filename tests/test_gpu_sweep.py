@@ -6,7 +6,10 @@ largest element). The same algorithm evaluated in float32 on the CPU is printed 
 far the case is conditioned at float32 precision (it misses these bars on scale gradients along the thin axis of
 needle-like primitives, which the tracer's FP64 root adjoint holds).
 Cases with a tmax that cuts through primitives and k-buffer sizes other than 16 check that results follow the
-reference's 16-event rounds whatever the buffer size."""
+reference's 16-event rounds whatever the buffer size. The scenes are traced in DC colour mode, where features of
+degree 3 colour events by their DC term and the backward is that forward's exact adjoint (both here and in the
+oracle): such cases exercise the degree-0 path with (N,16,3) inputs, not the SH evaluation (which has its own tests
+and the full-size configuration tests through trace_rays_sh)."""
 import math
 import os
 
