@@ -197,8 +197,6 @@ __device__ __forceinline__ double rsq_nr_d(double x) {
   r = r * fma(-0.5 * x * r, r, 1.5);
   return r * fma(-0.5 * x * r, r, 1.5);
 }
-// 1 / b' of safe_div's guarded denominator (b' = b, or kTiny where |b| < kTiny)
-__device__ __forceinline__ double safe_rcp_fast_d(double b) { return rcp_nr_d(fabs(b) < (double)kTiny ? (double)kTiny : b); }
 
 // The ray of the integrator in FP64: unit direction (l2 normalization of the raw direction); the origin is
 // rayo + tmin * rayd evaluated in FP64 (shaders.slang:178-179).
