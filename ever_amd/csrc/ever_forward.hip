@@ -123,9 +123,52 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   return v;
 }
 
-// Sort key of a ray: 30-bit Morton code of the point o + d * |c - o| (c = centre of the primitive centroids),
-// quantized over a cube of four times the scene extent around c. Rays of neighbouring pixels of one camera map
-// to neighbouring points, so a wave traces a compact screen tile; rays from different origins separate.
+__device__ __forceinline__ uint32_t expand14_2d(uint32_t v) {  // 14 bits -> every other bit of 28
+  v &= 0x3fffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// The reference camera of a ray batch for the sort keys: ray 0's origin o0, looking at the scene centre c (centre of
+// the primitive centroids), an orthonormal frame (u, v, m) with m along c - o0, and a ray's gnomonic coordinates
+// (dot(d, u), dot(d, v)) / dot(d, m) on the plane normal to m: for a pinhole camera at o0 an image of its pixel grid.
+struct KeyFrame {
+  f3 o0, u, v, m;
+  bool ok;  // the frame exists (o0 is not at the centre)
+};
+__device__ __forceinline__ KeyFrame key_frame(const float* rayo, f3 c, float ext) {
+  KeyFrame F;
+  F.o0 = ld3(rayo);
+  const f3 m = c - F.o0;
+  const float mn = sqrtf(dot3(m, m));
+  F.ok = mn > 1e-6f * ext;
+  F.m = F.ok ? (1.f / mn) * m : mk3(0.f, 0.f, 1.f);
+  // u from the coordinate axis least aligned with m
+  const f3 ax = fabsf(F.m.x) <= fabsf(F.m.y) && fabsf(F.m.x) <= fabsf(F.m.z) ? mk3(1.f, 0.f, 0.f)
+                : (fabsf(F.m.y) <= fabsf(F.m.z) ? mk3(0.f, 1.f, 0.f) : mk3(0.f, 0.f, 1.f));
+  F.u = l2n(cross3(ax, F.m));
+  F.v = cross3(F.m, F.u);
+  return F;
+}
+// gnomonic coordinates of a ray of the reference camera; false when the ray starts elsewhere or points more than
+// acos(0.25) away from m
+__device__ __forceinline__ bool key_plane(const KeyFrame& F, f3 ro, f3 d, float& a, float& b) {
+  const float dm = dot3(d, F.m);
+  if (!(F.ok && ro.x == F.o0.x && ro.y == F.o0.y && ro.z == F.o0.z && dm > 0.25f)) return false;
+  a = dot3(d, F.u) / dm;
+  b = dot3(d, F.v) / dm;
+  return true;
+}
+
+// Sort key of a ray (30 bits). A ray of the reference camera (above) is keyed by the 2D Morton code (14 bits per axis)
+// of its gnomonic coordinates on a grid whose cell is the spacing between rays 0 and 1 (neighbouring pixels of an image
+// in scanline order; at least 1/2048 of the plane's +-4 range): 64 consecutive keys then form a square screen tile of
+// 8 x 8 pixels (measured at 1920x1080: forward 8.6 -> 7.8 ms against the 3D key below). Other rays (other origins, wide
+// angles) follow, keyed by the 3D Morton code (29 bits) of the point o + d * |c - o|, quantized over a cube of four
+// times the scene extent around c, so that rays from different origins separate. Any order gives the same results.
 __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ rayo, const float* __restrict__ rayd,
                                                   int64_t nr, float tmin, const uint32_t* __restrict__ header,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -137,15 +180,30 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ rayo
   const f3 c = 0.5f * (lo + hi);
   const float ext = fmaxf(fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z), 1e-6f);
   const f3 rd = ld3(rayd + 3 * r);
-  const f3 o = ld3(rayo + 3 * r) + tmin * rd;
+  const f3 ro = ld3(rayo + 3 * r);
   const f3 d = l2n(rd);
-  const f3 co = c - o;
-  const f3 p = o + sqrtf(dot3(co, co)) * d;
-  const float inv = 1.f / (4.f * ext);
-  const uint32_t qx = (uint32_t)fminf(fmaxf(((p.x - c.x) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
-  const uint32_t qy = (uint32_t)fminf(fmaxf(((p.y - c.y) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
-  const uint32_t qz = (uint32_t)fminf(fmaxf(((p.z - c.z) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
-  keys[r] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+  const KeyFrame F = key_frame(rayo, c, ext);
+  float a, b;
+  if (key_plane(F, ro, d, a, b)) {
+    // grid cell: the spacing of rays 0 and 1 on the plane, within [8 / 16384, 8 / 256]
+    float cell = 8.f / 16384.f;
+    float a0, b0, a1, b1;
+    if (nr > 1 && key_plane(F, ld3(rayo), l2n(ld3(rayd)), a0, b0) && key_plane(F, ld3(rayo + 3), l2n(ld3(rayd + 3)), a1, b1))
+      cell = fminf(fmaxf(sqrtf((a1 - a0) * (a1 - a0) + (b1 - b0) * (b1 - b0)), 8.f / 16384.f), 8.f / 256.f);
+    const float inv = 1.f / cell;
+    const uint32_t qa = (uint32_t)fminf(fmaxf(a * inv + 8192.f, 0.f), 16383.f);
+    const uint32_t qb = (uint32_t)fminf(fmaxf(b * inv + 8192.f, 0.f), 16383.f);
+    keys[r] = (expand14_2d(qb) << 1) | expand14_2d(qa);
+  } else {
+    const f3 o = ro + tmin * rd;
+    const f3 co = c - o;
+    const f3 p = o + sqrtf(dot3(co, co)) * d;
+    const float inv = 1.f / (4.f * ext);
+    const uint32_t qx = (uint32_t)fminf(fmaxf(((p.x - c.x) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+    const uint32_t qy = (uint32_t)fminf(fmaxf(((p.y - c.y) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+    const uint32_t qz = (uint32_t)fminf(fmaxf(((p.z - c.z) * inv + 0.5f) * 1024.f, 0.f), 1023.f);
+    keys[r] = (1u << 29) | (((expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz)) >> 1);
+  }
   vals[r] = (uint32_t)r;
 }
 
