@@ -139,9 +139,9 @@ struct KeyFrame {
   f3 o0, u, v, m;
   bool ok;  // the frame exists (o0 is not at the centre)
 };
-__device__ __forceinline__ KeyFrame key_frame(const float* rayo, f3 c, float ext) {
+__device__ __forceinline__ KeyFrame key_frame(f3 o0, f3 c, float ext) {
   KeyFrame F;
-  F.o0 = ld3(rayo);
+  F.o0 = o0;
   const f3 m = c - F.o0;
   const float mn = sqrtf(dot3(m, m));
   F.ok = mn > 1e-6f * ext;
@@ -163,14 +163,29 @@ __device__ __forceinline__ bool key_plane(const KeyFrame& F, f3 ro, f3 d, float&
   return true;
 }
 
-// Sort key of a ray (30 bits). A ray of the reference camera (above) is keyed by the 2D Morton code (14 bits per axis)
-// of its gnomonic coordinates on a grid whose cell is the spacing between rays 0 and 1 (neighbouring pixels of an image
-// in scanline order; at least 1/2048 of the plane's +-4 range): 64 consecutive keys then form a square screen tile of
-// 8 x 8 pixels (measured at 1920x1080: forward 8.6 -> 7.8 ms against the 3D key below). Other rays (other origins, wide
-// angles) follow, keyed by the 3D Morton code (29 bits) of the point o + d * |c - o|, quantized over a cube of four
-// times the scene extent around c, so that rays from different origins separate. Any order gives the same results.
+// Sort key of a ray (30 bits). A ray of the reference camera (above) is keyed by the 2D Morton code of its gnomonic
+// coordinates on a grid whose cell is the spacing between the camera's first two rays (neighbouring pixels of an
+// image in scanline order; at least 1/2048 of the plane's +-4 range): 64 consecutive keys then form a square screen
+// tile of 8 x 8 pixels (measured at 1920x1080: forward 8.60 -> 7.11 ms against the 3D key below). With one camera the
+// reference is ray 0's origin and the code has 14 bits per axis; with several views (view_index) each view is its own
+// camera (its origin, its first two rays), the view's index leads the key and the code has (29 - view bits) / 2 bits
+// per axis. Other rays (other origins, wide angles) follow, keyed by the 3D Morton code (29 bits) of the point
+// o + d * |c - o|, quantized over a cube of four times the scene extent around c, so that rays from different origins
+// separate. Any order gives the same results.
+// first ray of each view's run of rays (the last run's, if a view's rays are not contiguous)
+__global__ __launch_bounds__(256) void k_view_first(const int32_t* __restrict__ view_index, int64_t nr, int32_t nv,
+                                                    int32_t* __restrict__ first) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int32_t v = view_index[r];
+  if (v >= 0 && v < nv && (r == 0 || view_index[r - 1] != v)) first[v] = (int32_t)r;
+}
+
 __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ rayo, const float* __restrict__ rayd,
                                                   int64_t nr, float tmin, const uint32_t* __restrict__ header,
+                                                  const int32_t* __restrict__ view_index,
+                                                  const float* __restrict__ view_origins, int32_t nv,
+                                                  const int32_t* __restrict__ view_first,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
@@ -182,18 +197,30 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ rayo
   const f3 rd = ld3(rayd + 3 * r);
   const f3 ro = ld3(rayo + 3 * r);
   const f3 d = l2n(rd);
-  const KeyFrame F = key_frame(rayo, c, ext);
+  // the ray's camera: its view's origin and first ray, or ray 0
+  const int32_t v = nv > 1 ? view_index[r] : 0;
+  const bool views = nv > 1 && v >= 0 && v < nv;
+  const int64_t r0 = views ? (int64_t)view_first[v] : 0;
+  const KeyFrame F = key_frame(views ? ld3(view_origins + 3 * (int64_t)v) : ld3(rayo), c, ext);
+  // view bits leading the key, and bits per axis of the 2D code
+  int vb = 0;
+  while (views && vb < 8 && (1 << vb) < nv) ++vb;
+  const int axb = (29 - vb) / 2 > 14 ? 14 : (29 - vb) / 2;
   float a, b;
   if (key_plane(F, ro, d, a, b)) {
-    // grid cell: the spacing of rays 0 and 1 on the plane, within [8 / 16384, 8 / 256]
+    // grid cell: the spacing of the camera's first two rays on the plane, within [8 / 16384, 8 / 256]
     float cell = 8.f / 16384.f;
     float a0, b0, a1, b1;
-    if (nr > 1 && key_plane(F, ld3(rayo), l2n(ld3(rayd)), a0, b0) && key_plane(F, ld3(rayo + 3), l2n(ld3(rayd + 3)), a1, b1))
+    if (r0 + 1 < nr && (!views || view_index[r0 + 1] == v) &&
+        key_plane(F, ld3(rayo + 3 * r0), l2n(ld3(rayd + 3 * r0)), a0, b0) &&
+        key_plane(F, ld3(rayo + 3 * (r0 + 1)), l2n(ld3(rayd + 3 * (r0 + 1))), a1, b1))
       cell = fminf(fmaxf(sqrtf((a1 - a0) * (a1 - a0) + (b1 - b0) * (b1 - b0)), 8.f / 16384.f), 8.f / 256.f);
     const float inv = 1.f / cell;
-    const uint32_t qa = (uint32_t)fminf(fmaxf(a * inv + 8192.f, 0.f), 16383.f);
-    const uint32_t qb = (uint32_t)fminf(fmaxf(b * inv + 8192.f, 0.f), 16383.f);
-    keys[r] = (expand14_2d(qb) << 1) | expand14_2d(qa);
+    const float half = (float)(1 << (axb - 1)), top = (float)((1 << axb) - 1);
+    const uint32_t qa = (uint32_t)fminf(fmaxf(a * inv + half, 0.f), top);
+    const uint32_t qb = (uint32_t)fminf(fmaxf(b * inv + half, 0.f), top);
+    const uint32_t code = (expand14_2d(qb) << 1) | expand14_2d(qa);  // 2 * axb bits
+    keys[r] = views ? ((uint32_t)(v & ((1 << vb) - 1)) << (2 * axb)) | code : code;
   } else {
     const f3 o = ro + tmin * rd;
     const f3 co = c - o;
@@ -1100,8 +1127,16 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
   uint32_t* k0 = at<uint32_t>(O.workspace, FL.keys0);
   uint32_t* v0 = at<uint32_t>(O.workspace, FL.vals0);
   if (R.num_rays >= kMinRaysToSort && P.num_prims > 0) {
+    // several views: each view keys its rays as its own camera; the first ray of each view goes to the sort's second
+    // value buffer, unused until the sort starts
+    const bool views = P.num_views > 1 && R.view_index && P.num_views <= R.num_rays;
+    int32_t* vfirst = at<int32_t>(O.workspace, FL.vals1);
+    if (views)
+      hipLaunchKernelGGL(k_view_first, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, R.view_index, R.num_rays,
+                         P.num_views, vfirst);
     hipLaunchKernelGGL(k_ray_keys, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, R.origins, R.directions,
-                       R.num_rays, R.tmin, header, k0, v0);
+                       R.num_rays, R.tmin, header, R.view_index, P.view_origins, views ? P.num_views : 0, vfirst, k0,
+                       v0);
     hipError_t e = launch_sort_pairs(k0, v0, at<uint32_t>(O.workspace, FL.keys1), at<uint32_t>(O.workspace, FL.vals1),
                                      at<uint32_t>(O.workspace, FL.hist), R.num_rays, 30, st);
     if (e != hipSuccess) return e;
