@@ -21,6 +21,7 @@ constexpr int kFwdBlock = 64;
 constexpr int kLdsDepth = kLdsStackPerRay;  // LDS stack entries per lane
 constexpr int kSpillDepth = kSpillPerRay;    // global spill entries per ray
 constexpr int kWaveBlock = 64;   // one wave per block in the per-wave traversal
+constexpr int64_t kXcdChunk = 16;  // consecutive 64-ray tiles given to one XCD (per-wave traversal)
 
 // Waves per SIMD that the per-wave walk is compiled for (the VGPR budget: 128 at 4 waves). The block's LDS (K events
 // of 8 B and a 4-B stack row per lane) admits 18 blocks per CU at K = 16 and 16 at K = 19.
@@ -1066,7 +1067,13 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
     if (base + lane < A.nr) ray_write(A, S);
   };
   if (!kPersistent) {
-    trace_tile((int64_t)blockIdx.x * kWaveBlock);
+    // blocks are dispatched to the 8 XCDs in turn (block b on XCD b % 8): give each XCD runs of kXcdChunk consecutive
+    // tiles (a 32x32 pixel block of one camera) so that its L2 serves neighbouring tiles (measured 1-2% faster than
+    // one tile per XCD at a time; runs of 256 tiles, 8% slower: the XCDs then stop sharing the last-level cache)
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t full = nb / (8 * kXcdChunk) * (8 * kXcdChunk);
+    const int64_t t = b < full ? ((b / 8) / kXcdChunk) * (8 * kXcdChunk) + (b % 8) * kXcdChunk + (b / 8) % kXcdChunk : b;
+    trace_tile(t * kWaveBlock);
     return;
   }
   // persistent: fetch A.tile consecutive tiles at a time until the batch is exhausted (every wave reaches the
