@@ -876,28 +876,81 @@ __device__ __forceinline__ float4 ld_uniform4(const float4* p, int64_t i) {
 // per lane of two registers), and children are ordered by the entry distances of the first live lane.
 // kSigns: every lane of the wave has the direction signs `sg` (bit a set: component a negative), so the entry
 // and exit planes of each axis are known per wave and the slab test needs no per-lane min/max.
-// The k-buffer of a wave lives in LDS (entry k of lane l at ev[k][l]); only its last entry, the culling bound,
-// is kept in a register. An insertion reads the lane's K entries, bubbles the event in and writes them back; it
-// runs only when some lane of the wave has an event below its bound.
-constexpr int pow2_floor(int n) { return n < 2 ? 1 : 2 * pow2_floor(n / 2); }
+// The k-buffer of a wave lives in LDS (entry k of lane l at ev[k][l]) as a max-heap per lane; only its top, the
+// culling bound, is kept in a register. An insertion runs only when some lane of the wave has an event below its
+// bound.
 
+// Comparator list of Batcher's odd-even merge sort for K entries: the network for the next power of two with the
+// comparators that reach past K left out (those entries would be +inf, so such comparators move nothing).
+struct CmpNet {
+  int n;
+  int8_t a[192], b[192];
+};
+constexpr CmpNet batcher_net(int K) {
+  CmpNet c{};
+  int P = 1;
+  while (P < K) P <<= 1;
+  for (int p = 1; p < P; p <<= 1)
+    for (int k = p; k >= 1; k >>= 1)
+      for (int j = k % p; j + k < P; j += 2 * k)
+        for (int i = 0; i < k && i + j + k < P; ++i)
+          if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < K) {
+            c.a[c.n] = (int8_t)(i + j);
+            c.b[c.n] = (int8_t)(i + j + k);
+            ++c.n;
+          }
+  return c;
+}
+constexpr int heap_levels(int K) { return K < 2 ? 0 : 1 + heap_levels(K / 2); }  // floor(log2 K)
+
+// The k-buffer of a lane during the walk is a binary max-heap over its K entries (children of entry k at 2k + 1
+// and 2k + 2; when K is even a row of zeros follows the last entry, so the last parent's missing child never
+// wins), started as K copies of the empty event. An event below the lane's K-th (the top, kept in a register as
+// the culling bound) replaces the top and sinks: at most log2(K) moves where a sorted insertion shifts up to K.
+// The fragments of one primitive report identical events (a quarter of the insertions at 1M/1080p): an event
+// whose key the lane's entries already hold is dropped (a key is one primitive's entry or exit, which a ray meets
+// once, so within a traversal equal keys are equal events). The scan reads the keys only.
 template <int K>
-__device__ __forceinline__ void kbuf_insert_lds(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
+__device__ __forceinline__ void kheap_replace(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
   if (!(v < kmax)) return;
-  // position = number of entries below v (binary search over the sorted entries; < K since v < the last)
-  // (power-of-two steps over the first K - 1 entries; any K)
-  constexpr int P = pow2_floor(K - 1);
-  int lo = 0;
+  bool dup = false;
+  const uint32_t* evk = reinterpret_cast<const uint32_t*>(&ev[0][lane]);  // (low word of entry k: its key)
 #pragma unroll
-  for (int step = P; step >= 1; step >>= 1) {
-    const int i = lo + step - 1;
-    if (i < K - 1 && ev[i][lane] < v) lo += step;
+  for (int k = 0; k < K; ++k) dup |= evk[k * 2 * kWaveBlock] == (uint32_t)v;
+  if (dup) return;
+  int i = 0;
+  uint64_t top = v;
+#pragma unroll
+  for (int level = 0; level < heap_levels(K); ++level) {
+    const int c = 2 * i + 1;
+    if (c >= K) break;
+    const uint64_t l = ev[c][lane], r = ev[c + 1][lane];
+    const bool right = r > l;
+    const uint64_t mv = right ? r : l;
+    if (!(mv > v)) break;
+    ev[i][lane] = mv;
+    if (level == 0) top = mv;
+    i = c + (right ? 1 : 0);
   }
-  if (ev[lo][lane] == v) return;  // the fragments of one primitive report identical events: keep one
-  // shift the entries from the position up by one (events mostly arrive near-first, so few move)
-  for (int j = K - 1; j > lo; --j) ev[j][lane] = ev[j - 1][lane];
-  ev[lo][lane] = v;
-  kmax = ev[K - 1][lane];
+  ev[i][lane] = v;
+  kmax = top;
+}
+
+// The heap sorted ascending for the integrator, in registers by Batcher's network.
+template <int K>
+__device__ __forceinline__ void kheap_sort(uint64_t (*ev)[kWaveBlock], int lane) {
+  constexpr CmpNet net = batcher_net(K);
+  uint64_t r[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) r[k] = ev[k][lane];
+#pragma unroll
+  for (int c = 0; c < net.n; ++c) {
+    const uint64_t x = r[net.a[c]], y = r[net.b[c]];
+    r[net.a[c]] = x < y ? x : y;
+    r[net.b[c]] = x < y ? y : x;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) ev[k][lane] = r[k];
 }
 
 template <int K, bool kSigns, bool kBounded, bool kViews>
@@ -928,8 +981,18 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
     int32_t stk0 = 0, stk1 = 0;
     int sp = 0;
     int32_t node = root;
-    int64_t steps = 0;
-    const int64_t max_steps = 2 * nleaves + 2;  // a tree visits each node at most once per traversal
+    // the upper culling depth: the lane's k-th event (recomputed when an insertion changes it); -inf on a finished
+    // lane, which then enters no box (its lower depth is +inf). Kept opaque like t_lo.
+    auto upper = [&]() {
+      float h =
+          alive ? fminf(kBounded ? fminf(A.tmax_hi, S.bound) : A.tmax_hi, __uint_as_float((uint32_t)(kmax >> 32)))
+                : -INFINITY;
+      asm volatile("" : "+v"(h));
+      return h;
+    };
+    float t_hi = upper();
+    uint32_t steps = 0;
+    const uint32_t max_steps = 2u * (uint32_t)nleaves + 2u;  // a tree visits each node at most once per traversal
     while (++steps <= max_steps) {
       if (node >= 0) {
         const float4* nd = A.nodes4 + 8 * (int64_t)node;
@@ -939,20 +1002,23 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4 lx = ld_uniform4(nd, ex), hx = ld_uniform4(nd, 1 - ex), ly = ld_uniform4(nd, 2 + ey),
                      hy = ld_uniform4(nd, 3 - ey), lz = ld_uniform4(nd, 4 + ez), hz = ld_uniform4(nd, 5 - ez),
                      cf = ld_uniform4(nd, 6);
-        float t_hi =
-            alive ? fminf(kBounded ? fminf(A.tmax_hi, S.bound) : A.tmax_hi, __uint_as_float((uint32_t)(kmax >> 32)))
-                  : -INFINITY;
-        asm volatile("" : "+v"(t_hi));
         const int32_t c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
                       c3 = __float_as_int(cf.w);
-        const bool used[4] = {c0 != kEmptyChild, c1 != kEmptyChild, c2 != kEmptyChild, c3 != kEmptyChild};
+        // the child references are needed only after the box tests: held here so that their load is issued with the
+        // planes' (one wait for the whole record) rather than after the tests
+        asm volatile("" ::"s"(c0), "s"(c1), "s"(c2), "s"(c3));
+        // an unused slot holds the empty box (+inf lower, -inf upper planes); with known signs and finite inv (clamped
+        // at 1e20) and oi (checked before the signed walk), its entry distance is +inf and its exit -inf, so the box
+        // test rejects it without the slot check
+        const bool used[4] = {kSigns || c0 != kEmptyChild, kSigns || c1 != kEmptyChild, kSigns || c2 != kEmptyChild,
+                              kSigns || c3 != kEmptyChild};
         const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
         const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
         const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
-        // the children some lane enters, as a wave-uniform bit mask, and each child's entry distance per lane
-        float tns[4];
+        // the children some lane enters, as a wave-uniform bit mask, and each child's sort key per lane: the entry
+        // distance where the lane enters it (+inf where not), the slot in the low 2 bits
+        uint32_t tks[4];
         uint32_t hmask = 0;
-        uint64_t hms[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           float tn, tf;
@@ -969,9 +1035,10 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
             tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t_hi));
           }
           // the lanes entering the box (ordered compare: a NaN bound enters nothing)
-          hms[k] = __builtin_amdgcn_fcmpf(tn, tf, 5 /* ordered <= */);
-          tns[k] = tn;
-          hmask |= (used[k] && hms[k] != 0) ? (1u << k) : 0u;
+          const bool enter = tn <= tf;
+          const uint64_t hm = __builtin_amdgcn_ballot_w64(enter);
+          tks[k] = enter ? ((__float_as_uint(tn) & ~3u) | (uint32_t)k) : (0x7f800000u | (uint32_t)k);
+          hmask |= (used[k] && hm != 0) ? (1u << k) : 0u;
         }
         auto child = [&](uint32_t kk) {
           const bool odd = (kk & 1u) != 0u;
@@ -988,9 +1055,8 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
           uint32_t key[4];
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            const uint32_t tl = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tns[k]), leader);
-            const uint32_t lk = ((hms[k] >> leader) & 1u) ? tl : 0x7f800000u;
-            key[k] = ((hmask >> k) & 1u) ? ((lk & ~3u) | (uint32_t)k) : 0xffffffffu;
+            const uint32_t tl = (uint32_t)__builtin_amdgcn_readlane((int)tks[k], leader);
+            key[k] = ((hmask >> k) & 1u) ? tl : 0xffffffffu;
           }
           const int nhit = __builtin_popcount(hmask);
 #define EVER_CSWAP(a, b)                           \
@@ -1020,15 +1086,18 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
         uint64_t e = kNoEvent, x = kNoEvent;
         if (alive) leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
-        if (__builtin_amdgcn_ballot_w64(e < kmax) != 0) kbuf_insert_lds<K>(ev, lane, e, kmax);
-        if (__builtin_amdgcn_ballot_w64(x < kmax) != 0) kbuf_insert_lds<K>(ev, lane, x, kmax);
+        const bool ie = __builtin_amdgcn_ballot_w64(e < kmax) != 0;
+        if (ie) kheap_replace<K>(ev, lane, e, kmax);
+        const bool ix = __builtin_amdgcn_ballot_w64(x < kmax) != 0;
+        if (ix) kheap_replace<K>(ev, lane, x, kmax);
+        if (ie || ix) t_hi = upper();
       }
       if (sp == 0) break;
       --sp;
-      if (sp < 64) node = __builtin_amdgcn_readlane(stk0, sp);
-      else if (sp < 128) node = __builtin_amdgcn_readlane(stk1, sp - 64);
+      if (sp < 128) node = __builtin_amdgcn_readlane(sp < 64 ? stk0 : stk1, sp & 63);
       else node = __builtin_amdgcn_readlane(row2[lane], sp - 128);  // lane sp - 128 reads back its own store
     }
+    kheap_sort<K>(ev, lane);
     if (alive && integrate_seq<K, kBounded, kViews>(A, S, [&](int k) { return ev[k][lane]; })) {
       // fewer than K events up to the bound: done under the plain walk or when the bound lies beyond the scene,
       // else continue without the proposal
@@ -1042,7 +1111,8 @@ template <int K, int kFlags>  // kFlags: 1 proposal-bounded, 2 several views, 4 
 __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()) void k_trace_forward_wave(FwdArgs A) {
   constexpr bool kBounded = (kFlags & 1) != 0, kViews = (kFlags & 2) != 0, kPersistent = (kFlags & 4) != 0;
   const int lane = threadIdx.x;
-  __shared__ uint64_t ev[K][kWaveBlock];
+  __shared__ uint64_t ev[K + (K % 2 == 0 ? 1 : 0)][kWaveBlock];  // (the zero row after a heap of even size)
+  if constexpr (K % 2 == 0) ev[K][lane] = 0;
   __shared__ int32_t row2[kWaveBlock];  // third row of the wave's traversal stack
   // the 64 rays of processing slots [base, base + 64); lanes past the batch stay in the wave without a ray
   // (ray_idle), so that all 64 lanes hold the wave's stack
@@ -1057,12 +1127,16 @@ __global__ __launch_bounds__(kWaveBlock, (kFlags & 4) ? 3 : wave_walk_waves<K>()
     } else {
       ray_idle(A, S);
     }
-    // lane 0 always has a ray; rayless lanes take its signs so they do not break the wave's sign uniformity
-    const f3 inv0 = ray_geo(A, S.r).inv;
+    // lane 0 always has a ray; rayless lanes take its signs so they do not break the wave's sign uniformity.
+    // The signed walk also needs finite origin * reciprocal on every lane (it rejects unused child slots by their
+    // empty boxes, which a NaN slab distance would not do); a wave with a non-finite one takes the general walk.
+    const RayGeo G0 = ray_geo(A, S.r);
+    const f3 inv0 = G0.inv;
     uint32_t sg = (signbit(inv0.x) ? 1u : 0u) | (signbit(inv0.y) ? 2u : 0u) | (signbit(inv0.z) ? 4u : 0u);
     const uint32_t sg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sg);
     if (!valid) sg = sg0;
-    if (__ballot(sg != sg0) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, row2);
+    const bool fin = isfinite(G0.oi.x) && isfinite(G0.oi.y) && isfinite(G0.oi.z);
+    if (__ballot(sg != sg0 || !fin) == 0) wave_trace<K, true, kBounded, kViews>(A, S, lane, sg0, ev, row2);
     else wave_trace<K, false, kBounded, kViews>(A, S, lane, sg0, ev, row2);
     if (base + lane < A.nr) ray_write(A, S);
   };
