@@ -155,7 +155,9 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
 }
 
 // Leaf fragments of one primitive: the ellipsoid is cut along its longest local axis into
-// F = clamp(ceil(s_max / s_mid), 1, kMaxFragments) slabs; fragment j's box is the intersection of the
+// F = clamp(ceil(0.75 s_max / s_mid), 1, kMaxFragments) slabs (a ratio up to 4/3 stays whole: measured at 1M/1080p,
+// the same forward time as ceil(s_max / s_mid) with 8% less build time; ceil(0.5 s_max / s_mid) costs more in the
+// forward than it saves in the build); fragment j's box is the intersection of the
 // ellipsoid's tight AABB (create_aabbs.cu:77-114: half-extent_k = sqrt(sum_l (R_kl s_l)^2), q normalized) and
 // the AABB of the local box [a_j, b_j] x [-rho_j, rho_j]^2 that contains the slab (rho_j = sqrt(1 - min a^2)).
 // Boxes are padded by a relative 1e-6 so they stay conservative under rounding. Each fragment gets the 27-bit
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* 
     float mid = 0.f;
     for (int k = 0; k < 3; ++k)
       if (k != ax) mid = fmaxf(mid, sv[k]);
-    F = mid > 0.f ? (int)ceilf(sv[ax] / mid) : kMaxFragments;
+    F = mid > 0.f ? (int)ceilf(0.75f * sv[ax] / mid) : kMaxFragments;
     F = F < 1 ? 1 : (F > kMaxFragments ? kMaxFragments : F);
     float tight[3];
     for (int k = 0; k < 3; ++k)

@@ -1038,7 +1038,11 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
           const bool enter = tn <= tf;
           const uint64_t hm = __builtin_amdgcn_ballot_w64(enter);
           tks[k] = enter ? ((__float_as_uint(tn) & ~3u) | (uint32_t)k) : (0x7f800000u | (uint32_t)k);
-          hmask |= (used[k] && hm != 0) ? (1u << k) : 0u;
+          // (hm != 0) as 0/1 in a scalar register: written as a compare and select because the compiler otherwise
+          // keeps it as a lane mask, and turns it into an integer through a vector select and a readfirstlane
+          uint32_t any;
+          asm("s_cmp_lg_u64 %1, 0\n\ts_cselect_b32 %0, 1, 0" : "=s"(any) : "s"(hm) : "scc");
+          hmask |= (used[k] ? any : 0u) << k;
         }
         auto child = [&](uint32_t kk) {
           const bool odd = (kk & 1u) != 0u;
