@@ -910,14 +910,10 @@ constexpr int heap_levels(int K) { return K < 2 ? 0 : 1 + heap_levels(K / 2); } 
 // The fragments of one primitive report identical events (a quarter of the insertions at 1M/1080p): an event
 // whose key the lane's entries already hold is dropped (a key is one primitive's entry or exit, which a ray meets
 // once, so within a traversal equal keys are equal events). The scan reads the keys only.
+
+// v replaces the top (v below it) and sinks to its place.
 template <int K>
-__device__ __forceinline__ void kheap_replace(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
-  if (!(v < kmax)) return;
-  bool dup = false;
-  const uint32_t* evk = reinterpret_cast<const uint32_t*>(&ev[0][lane]);  // (low word of entry k: its key)
-#pragma unroll
-  for (int k = 0; k < K; ++k) dup |= evk[k * 2 * kWaveBlock] == (uint32_t)v;
-  if (dup) return;
+__device__ __forceinline__ void kheap_replace_sift(uint64_t (*ev)[kWaveBlock], int lane, uint64_t v, uint64_t& kmax) {
   int i = 0;
   uint64_t top = v;
 #pragma unroll
@@ -934,6 +930,25 @@ __device__ __forceinline__ void kheap_replace(uint64_t (*ev)[kWaveBlock], int la
   }
   ev[i][lane] = v;
   kmax = top;
+}
+// A leaf's entry e and exit x (x > e, kNoEvent where none) inserted in turn, with one key scan for both (their keys
+// differ, so the scan before the entry's insertion answers for the exit too; one scan per leaf instead of one per
+// event: forward 6.69 -> 6.50 ms at 1M/1080p).
+template <int K>
+__device__ __forceinline__ void kheap_insert2(uint64_t (*ev)[kWaveBlock], int lane, uint64_t e, uint64_t x,
+                                              uint64_t& kmax) {
+  const bool te = e < kmax, tx = x < kmax;
+  if (!te && !tx) return;
+  bool de = false, dx = false;
+  const uint32_t* evk = reinterpret_cast<const uint32_t*>(&ev[0][lane]);  // (low word of entry k: its key)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t kk = evk[k * 2 * kWaveBlock];
+    de |= kk == (uint32_t)e;
+    dx |= kk == (uint32_t)x;
+  }
+  if (te && !de) kheap_replace_sift<K>(ev, lane, e, kmax);
+  if (x < kmax && !dx) kheap_replace_sift<K>(ev, lane, x, kmax);
 }
 
 // The heap sorted ascending for the integrator, in registers by Batcher's network.
@@ -1090,11 +1105,10 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
         uint64_t e = kNoEvent, x = kNoEvent;
         if (alive) leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
-        const bool ie = __builtin_amdgcn_ballot_w64(e < kmax) != 0;
-        if (ie) kheap_replace<K>(ev, lane, e, kmax);
-        const bool ix = __builtin_amdgcn_ballot_w64(x < kmax) != 0;
-        if (ix) kheap_replace<K>(ev, lane, x, kmax);
-        if (ie || ix) t_hi = upper();
+        if (__builtin_amdgcn_ballot_w64(e < kmax || x < kmax) != 0) {
+          kheap_insert2<K>(ev, lane, e, x, kmax);
+          t_hi = upper();
+        }
       }
       if (sp == 0) break;
       --sp;
