@@ -468,8 +468,9 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
   I.iter = 0;
   I.hold = 0;
   ist_store(A, slot, I);
-  // cursor over (t, key): initially nothing at t == 0 (the reference requires t > t_cur = 0)
-  S.cursor = pack_event(0.f, 0xffffffffu);
+  // cursor over (t, key): below every event (events at float32 depths down to the start band are packed at t = 0
+  // and decided from their FP64 depths, which the reference requires to be > t_cur = 0; see leaf_events)
+  S.cursor = 0;
   S.bound = INFINITY;
   S.tref = band_lo(0.f);
   S.live = A.max_iters > 0;
@@ -538,13 +539,17 @@ __device__ __forceinline__ void leaf_events(const FwdArgs& A, const RayState& S,
   const float qq = fmaf(sgnf(bp), __builtin_amdgcn_sqrtf(fmaxf(h, 0.f)), bp);
   const float r0 = cc * __builtin_amdgcn_rcpf(qq), r1 = qq * ia;
   const float t0 = fminf(r0, r1), t1 = fmaxf(r0, r1);
-  const uint64_t ve = pack_event(t0, 2u * s), vx = pack_event(t1, 2u * s + 1u);
+  // at the ray's start (cursor depth 0) events down to the gathering band below it are gathered too, packed at depth
+  // 0: the float32 root of a surface through the start point can fall on either side of it, and the integrator
+  // decides from the FP64 depth, which the reference requires to be > 0
+  const float lo_t = cur_t(S) > 0.f ? cur_t(S) : band_lo(0.f);
+  const uint64_t ve = pack_event(fmaxf(t0, 0.f), 2u * s), vx = pack_event(fmaxf(t1, 0.f), 2u * s + 1u);
   // a miss within the grazing band still enters the buffer (with the roots of 1 - |l|^2 clamped at 0): the
   // integrator decides such events from the raw parameters in FP64 and skips the pair if the ray misses
   const float bb = fmaf(RC.w, RC.w, fmaf(RB.w, RB.w, RA.w * RA.w));
   const bool meets = h >= 0.f || omll * omll < kGraze2 * (cc + 1.f + 4.f * bb);
-  const bool hit = meets && !(t1 < cur_t(S));
-  const bool with_entry = t0 >= cur_t(S) && ve > S.cursor;
+  const bool hit = meets && !(t1 < lo_t);
+  const bool with_entry = t0 >= lo_t && ve > S.cursor;
   const float tmax = A.tmax_hi;
   const bool ins_e = hit && with_entry && t0 <= tmax;
   // an exit beyond tmax enters only with its entry: in the reference's trace of the current round (from tref) the
@@ -619,6 +624,9 @@ __device__ __forceinline__ bool integrate_seq(const FwdArgs& A, RayState& S, Hea
       if (t0 > (double)A.tmax) continue;  // an entry beyond tmax, or an exit whose entry is beyond it
       const bool entry = (head & 1u) == 0u;
       nt = entry ? t0 : t1;
+      // the reference takes only events after the last one it processed (t > t_cur, shaders.slang:146; t_cur = 0 at
+      // the start): one gathered from the start band whose FP64 depth is not > 0 is not an event of its trace
+      if (!(nt > 0.0)) continue;
       nt0 = (entry || t1 <= (double)A.tmax) ? -INFINITY : t0;
       nh = head;
       got = true;
@@ -778,7 +786,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_trace_forward(FwdArgs A) {
     for (int k = 0; k < K; ++k) bk[k] = pack_event(kEmptyT, 0xffffffffu);
     // the lower culling depth is the cursor, held at tmax once the cursor has passed it: exits beyond tmax whose
     // entries lie before it may remain, and their boxes straddle tmax
-    const float t_lo = fminf(cur_t(S), A.tmax_lo);
+    const float t_lo = fminf(cur_t(S) > 0.f ? cur_t(S) : band_lo(0.f), A.tmax_lo);  // (the start band: leaf_events)
     // ---- traversal: gather the K nearest events after the cursor ----
     int sp = 0;
     int32_t node = root;
@@ -987,7 +995,7 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
     uint64_t kmax = kNoEvent;
     // a finished lane never enters a box; the cursor is held at tmax once past it (exits beyond tmax whose entries
     // lie before it may remain, in boxes that straddle tmax)
-    float t_lo = alive ? fminf(cur_t(S), A.tmax_lo) : INFINITY;
+    float t_lo = alive ? fminf(cur_t(S) > 0.f ? cur_t(S) : band_lo(0.f), A.tmax_lo) : INFINITY;  // (leaf_events)
     // kept opaque: otherwise the select is folded into every box test below (two extra selects per child)
     asm volatile("" : "+v"(t_lo));
     // wave stack: entry i in lane i % 64 of row i / 64; rows 0 and 1 in VGPRs, row 2 in LDS (entries beyond 128,
@@ -996,6 +1004,12 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
     int32_t stk0 = 0, stk1 = 0;
     int sp = 0;
     int32_t node = root;
+    // the last 64 primitives whose leaves the wave tested in this traversal (lane i holds entry i, replaced in turn):
+    // another fragment of one of them is skipped, since its test would give every lane the same events, which the
+    // lane then already holds or has rejected against a bound that has only come down since (measured 1.5-2% of the
+    // forward at 1M/1080p; 128 entries in two registers, no better)
+    uint32_t seen = 0xffffffffu;
+    uint32_t nseen = 0;
     // the upper culling depth: the lane's k-th event (recomputed when an insertion changes it); -inf on a finished
     // lane, which then enters no box (its lower depth is +inf). Kept opaque like t_lo.
     auto upper = [&]() {
@@ -1101,13 +1115,17 @@ __device__ __forceinline__ void wave_trace(const FwdArgs& A, RayState& S, int la
         }
       } else {
         const uint32_t s = (uint32_t)(~node);
-        const float4* rc = A.rec + 4 * (int64_t)s;
-        const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
-        uint64_t e = kNoEvent, x = kNoEvent;
-        if (alive) leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
-        if (__builtin_amdgcn_ballot_w64(e < kmax || x < kmax) != 0) {
-          kheap_insert2<K>(ev, lane, e, x, kmax);
-          t_hi = upper();
+        if (__builtin_amdgcn_ballot_w64(seen == s) == 0) {
+          seen = (lane == (int)(nseen & 63u)) ? s : seen;
+          ++nseen;
+          const float4* rc = A.rec + 4 * (int64_t)s;
+          const float4 RA = ld_uniform4(rc, 0), RB = ld_uniform4(rc, 1), RC = ld_uniform4(rc, 2);
+          uint64_t e = kNoEvent, x = kNoEvent;
+          if (alive) leaf_events<kBounded>(A, S, G, RA, RB, RC, s, e, x);
+          if (__builtin_amdgcn_ballot_w64(e < kmax || x < kmax) != 0) {
+            kheap_insert2<K>(ev, lane, e, x, kmax);
+            t_hi = upper();
+          }
         }
       }
       if (sp == 0) break;

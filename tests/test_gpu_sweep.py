@@ -69,8 +69,9 @@ def _draw(seed):
 
 
 # seeds beyond the first 96 that once failed: grazing hits, near-tied events ordered differently by float32 and FP64
-# depths, exits beyond a cutting tmax at the reference's round boundaries, thin-axis scale gradients
-HARD_SEEDS = (138, 156, 180, 204, 289, 311, 330, 342, 346, 348, 354, 369, 438, 468, 494, 496, 513, 522, 546)
+# depths, exits beyond a cutting tmax at the reference's round boundaries, thin-axis scale gradients, a camera inside
+# a primitive whose surface passes within 1e-8 of the start point (float32 and FP64 roots on opposite sides of it)
+HARD_SEEDS = (138, 156, 180, 204, 289, 311, 330, 342, 346, 348, 354, 369, 438, 468, 494, 496, 513, 522, 546, 1483)
 
 
 def _seeds():
