@@ -401,18 +401,11 @@ __global__ __launch_bounds__(256) void k_init_adjoint(const float* __restrict__ 
   const float4 gv = reinterpret_cast<const float4*>(dinit)[r];
   if (gv.x == 0.f && gv.y == 0.f && gv.z == 0.f && gv.w == 0.f) return;
   const int32_t view = vcol ? view_index[r] : 0;
-  const f3 p0 = ld3(vcol ? view_origins + 3 * view : rayo) + tmin * ld3(rayd + 3 * r);
+  const d3 p0 = tod(ld3(vcol ? view_origins + 3 * view : rayo)) + (double)tmin * tod(ld3(rayd + 3 * r));  // (FP64)
   for (int32_t j = 0; j < ncand; ++j) {
     const int64_t p = cand[j];
     if (vcol && ((reinterpret_cast<const uint64_t*>(cand + cand_mask_offset(n))[j] >> view) & 1u) == 0) continue;
-    float4 q = reinterpret_cast<const float4*>(quats)[p];
-    const float qn = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    q = make_float4(safe_div(q.x, qn), safe_div(q.y, qn), safe_div(q.z, qn), safe_div(q.w, qn));
-    const f3 v = qrot(p0 - ld3(means + 3 * p), q);
-    const f3 s = ld3(scales + 3 * p);
-    const f3 tr = mk3(safe_div(v.x, fmaxf(s.x, 1e-8f)), safe_div(v.y, fmaxf(s.y, 1e-8f)),
-                      safe_div(v.z, fmaxf(s.z, 1e-8f)));
-    if (sqrtf(dot3(tr, tr)) <= 1.f) {
+    if (inside_d(p0, ld3(means + 3 * p), ld3(scales + 3 * p), reinterpret_cast<const float4*>(quats)[p], true)) {
       const int64_t kv = vcol ? (int64_t)view * n + p : p;
       const float4 cr = vcol ? vcol[kv] : colr[p];
       float* ac = acc + 16 * p;

@@ -185,6 +185,30 @@ __device__ __forceinline__ d3 qrot_d(d3 v, float4 q) {
   return v + (double)q.x * t + crossd(u, t);
 }
 
+// Start-inside containment in FP64 (the point of a surface through the start is decided at the oracle's precision):
+// p - mean rotated by the quaternion normalised by its length, divided by the scales, squared length <= 1.
+// `adjoint`: the form of backwards_kernel.slang:395-405 (guarded divisions, scales clipped at 1e-8, length <= 1);
+// else initialize_density.cu:107-150.
+__device__ __forceinline__ double safe_div_d(double a, double b) {
+  return fmax(fmin(a / (fabs(b) < (double)kTiny ? (double)kTiny : b), (double)kMaxVal), (double)kMinVal);
+}
+__device__ __forceinline__ bool inside_d(d3 p, f3 mean, f3 s, float4 q, bool adjoint) {
+  const double qn = sqrt((double)q.x * q.x + (double)q.y * q.y + (double)q.z * q.z + (double)q.w * q.w);
+  const double w = adjoint ? safe_div_d(q.x, qn) : q.x / qn, x = adjoint ? safe_div_d(q.y, qn) : q.y / qn,
+               y = adjoint ? safe_div_d(q.z, qn) : q.z / qn, z = adjoint ? safe_div_d(q.w, qn) : q.w / qn;
+  const d3 v = p - tod(mean);
+  const d3 u = mkd(-x, -y, -z);
+  const d3 t = 2.0 * crossd(u, v);
+  const d3 r = v + w * t + crossd(u, t);
+  if (adjoint) {
+    const d3 tr = mkd(safe_div_d(r.x, fmax((double)s.x, 1e-8)), safe_div_d(r.y, fmax((double)s.y, 1e-8)),
+                      safe_div_d(r.z, fmax((double)s.z, 1e-8)));
+    return sqrt(dotd(tr, tr)) <= 1.0;
+  }
+  const d3 tr = mkd(r.x / s.x, r.y / s.y, r.z / s.z);
+  return dotd(tr, tr) <= 1.0;
+}
+
 // FP64 reciprocal and reciprocal square root from the hardware estimates (v_rcp_f64, v_rsq_f64) refined by two
 // Newton steps: within a few ulp of the IEEE operations.
 __device__ __forceinline__ double rcp_nr_d(double b) {

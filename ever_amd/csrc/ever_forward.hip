@@ -425,22 +425,18 @@ __device__ __forceinline__ void ray_setup(const FwdArgs& A, int64_t r, int64_t s
   S.e = (int32_t)(A.events_by_slot ? slot : r);
   S.slot = (int32_t)slot;
   const f3 ro = ld3(A.rayo + 3 * r);
-  // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150)
+  // initial density / colour of ellipsoids containing rayo + tmin * d (initialize_density.cu:107-150), decided in FP64
+  // (a start point on a surface is placed as the oracle places it)
   double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
   const int32_t ncand = (int32_t)A.header[8];
-  const f3 p = ro + A.tmin * l2n(ld3(A.rayd + 3 * r));
+  const d3 rdd = tod(ld3(A.rayd + 3 * r));
+  const d3 p = tod(ro) + ((double)A.tmin / sqrt(dotd(rdd, rdd))) * rdd;  // (FP64, as the oracle)
   const int32_t view = kViews ? A.view_index[r] : 0;
   for (int32_t j = 0; j < ncand; ++j) {
     const int64_t i = A.cand[j];
     // several views: only the candidates of this ray's view
     if (kViews && ((reinterpret_cast<const uint64_t*>(A.cand + cand_mask_offset(A.n))[j] >> view) & 1u) == 0) continue;
-    float4 q = reinterpret_cast<const float4*>(A.quats)[i];
-    float qn = 1.f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    q = make_float4(q.x * qn, q.y * qn, q.z * qn, q.w * qn);
-    f3 v = qrot(p - ld3(A.means + 3 * i), q);
-    f3 sc = ld3(A.scales + 3 * i);
-    f3 tr = mk3(v.x / sc.x, v.y / sc.y, v.z / sc.z);
-    if (dot3(tr, tr) <= 1.f) {
+    if (inside_d(p, ld3(A.means + 3 * i), ld3(A.scales + 3 * i), reinterpret_cast<const float4*>(A.quats)[i], false)) {
       float den = A.dens[i];
       float c0, c1, c2;
       if (A.color_mode == EVER_COLOR_SH) {

@@ -62,6 +62,9 @@ def _draw(seed):
         K[0, 0] = K[1, 1] = 40.0
         wcts = (W @ K).astype(np.float32)[None]
     g = torch.from_numpy(rng.normal(size=(o.shape[0], 5)).astype(np.float32))
+    # the tracer takes tmin and tmax in float32: the oracle gets the same values (a start point within 1e-8 of a
+    # surface is placed differently by 0.05 and its float32 rounding)
+    tmin, tmax = float(np.float32(tmin)), float(np.float32(tmax))
     return dict(sc=sc, o=o, d=d, tmin=tmin, tmax=tmax, max_iters=max_iters, kbuf=kbuf, traversal=traversal, g=g,
                 wcts=wcts,
                 desc=f"seed {seed}: {kind} N={sc['means'].shape[0]} deg {deg} inside {inside} tmin {tmin} "
@@ -70,8 +73,10 @@ def _draw(seed):
 
 # seeds beyond the first 96 that once failed: grazing hits, near-tied events ordered differently by float32 and FP64
 # depths, exits beyond a cutting tmax at the reference's round boundaries, thin-axis scale gradients, a camera inside
-# a primitive whose surface passes within 1e-8 of the start point (float32 and FP64 roots on opposite sides of it)
-HARD_SEEDS = (138, 156, 180, 204, 289, 311, 330, 342, 346, 348, 354, 369, 438, 468, 494, 496, 513, 522, 546, 1483)
+# a primitive whose surface passes within 1e-8 of the start point (float32 and FP64 roots, or containment tests, on
+# opposite sides of it)
+HARD_SEEDS = (138, 156, 180, 204, 289, 311, 330, 342, 346, 348, 354, 369, 438, 468, 494, 496, 513, 522, 546, 1483,
+              2059)
 
 
 def _seeds():
