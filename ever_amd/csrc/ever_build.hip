@@ -31,7 +31,20 @@ __global__ void k_build_init(uint32_t* header) {
   else if (t < 16) header[t] = 0u;
 }
 
-// Per primitive (original order): colour record, traversal record and centroid bounds.
+// Leaf fragments of a primitive with scales sv: ceil(s_max / s_mid) clamped to [1, kMaxFragments]; ax is the axis
+// of s_max (the split axis). The prepass counts with it and the fragment emission emits with it.
+__device__ __forceinline__ int fragment_count(const float sv[3], int& ax) {
+  ax = 0;
+  if (sv[1] > sv[ax]) ax = 1;
+  if (sv[2] > sv[ax]) ax = 2;
+  float mid = 0.f;
+  for (int k = 0; k < 3; ++k)
+    if (k != ax) mid = fmaxf(mid, sv[k]);
+  const int F = mid > 0.f ? (int)ceilf(sv[ax] / mid) : kMaxFragments;
+  return F < 1 ? 1 : (F > kMaxFragments ? kMaxFragments : F);
+}
+
+// Per primitive (original order): colour record, traversal record, centroid bounds and the block's fragment count.
 //   record (64 B): rows (M_k, b_k), k = 0..2, with M = diag(1/s) L_q, b = -M mean, so that the ellipsoid-frame
 //   origin is ocn = M o + b and direction rdn = M d; L_q is the linear map of rotate_vector (safe-math.slang:237-244)
 //   for the raw quaternion, as the reference's intersection uses it (tri-intersect.slang:317-329).
@@ -40,7 +53,8 @@ __global__ void k_build_init(uint32_t* header) {
 // features on an unchanged geometry (ever_update_colors)
 template <bool kColorsOnly>
 __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __restrict__ rec,
-                                                      float4* __restrict__ colr, float* __restrict__ bpart) {
+                                                      float4* __restrict__ colr, float* __restrict__ bpart,
+                                                      uint32_t* __restrict__ fcount) {
   // the block's SH features staged through LDS with contiguous loads (a lane reading its own primitive's
   // coefficients would touch a different line per lane and load), 49 words per primitive so the lanes' reads
   // fall in distinct banks
@@ -50,21 +64,41 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
   const int per = 3 * P.feature_size;
   const bool staged = P.color_mode == EVER_COLOR_SH && per <= 48;
   if (staged) {
-    // block-local indices in 32 bits (a 64-bit division per element costs more than the load)
+    // block-local indices in 32 bits; twelve loads in flight per thread (a whole SH3 block in one pass) before
+    // their LDS stores, and the primitive of a word from a float reciprocal (idx < 12288 and per <= 48 keep
+    // (idx + 0.5) / per at least 0.01 from an integer, far above the rounding error)
     const int nb = P.num_prims - i0 < 256 ? (int)(P.num_prims - i0) : 256;
+    const int total = nb * per;
     const float* src = P.features + per * i0;
-    for (int idx = threadIdx.x; idx < nb * per; idx += blockDim.x) {
-      const int pl = idx / per;
-      fs[pl * kStride + (idx - pl * per)] = src[idx];
+    const float rper = 1.f / (float)per;
+    for (int b0 = 0; b0 < total; b0 += 12 * 256) {
+      float w[12];
+#pragma unroll
+      for (int u = 0; u < 12; ++u) {
+        const int idx = b0 + u * 256 + (int)threadIdx.x;
+        w[u] = idx < total ? src[idx] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 12; ++u) {
+        const int idx = b0 + u * 256 + (int)threadIdx.x;
+        const int pl = (int)(((float)idx + 0.5f) * rper);
+        if (idx < total) fs[pl * kStride + (idx - pl * per)] = w[u];
+      }
     }
     __syncthreads();
   }
   int64_t i = i0 + threadIdx.x;
   bool valid = i < P.num_prims;
   float cx = 0.f, cy = 0.f, cz = 0.f;
+  int nfrag = 0;
   if (valid) {
     const float4 q = reinterpret_cast<const float4*>(P.quats)[i];
     const f3 s = ld3(P.scales + 3 * i);
+    if (!kColorsOnly) {
+      const float sv[3] = {s.x, s.y, s.z};
+      int ax;
+      nfrag = fragment_count(sv, ax);
+    }
     const f3 m = ld3(P.means + 3 * i);
     cx = m.x; cy = m.y; cz = m.z;
     float dens = P.densities[i];
@@ -112,25 +146,34 @@ __global__ __launch_bounds__(256) void k_prim_prepass(ever_prims P, float4* __re
     }
   }
   if (kColorsOnly) return;
-  // block reduction of centroid bounds into a per-block partial (reduced by k_bounds_final)
+  // block reduction of centroid bounds into a per-block partial (reduced by k_bounds_final), and of the fragment
+  // count (scanned into fragment offsets by k_bounds_final)
   __shared__ float red[6][4];
+  __shared__ uint32_t nred[4];
   float v[6] = {valid ? cx : INFINITY, valid ? cy : INFINITY, valid ? cz : INFINITY,
                 valid ? -cx : INFINITY, valid ? -cy : INFINITY, valid ? -cz : INFINITY};
-  for (int o = 32; o > 0; o >>= 1)
+  for (int o = 32; o > 0; o >>= 1) {
     for (int k = 0; k < 6; ++k) v[k] = fminf(v[k], __shfl_xor(v[k], o));
+    nfrag += __shfl_xor(nfrag, o);
+  }
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
+  if ((threadIdx.x & 63) == 0) {
     for (int k = 0; k < 6; ++k) red[k][w] = v[k];
+    nred[w] = (uint32_t)nfrag;
+  }
   __syncthreads();
   if (threadIdx.x < 6) {
     const int k = threadIdx.x;
     bpart[6 * (int64_t)blockIdx.x + k] = fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3]));
   }
+  if (threadIdx.x == 0) fcount[blockIdx.x] = nred[0] + nred[1] + nred[2] + nred[3];
 }
 
-// Final reduction of the per-block bounds (one block); writes the ordered-int bounds into the header.
+// Final reduction of the per-block bounds (one block); writes the ordered-int bounds into the header. Also turns
+// the per-block fragment counts into exclusive offsets (in place) and writes their total, the leaf count L, into
+// header[9].
 __global__ __launch_bounds__(256) void k_bounds_final(const float* __restrict__ bpart, int64_t nb,
-                                                      uint32_t* __restrict__ header) {
+                                                      uint32_t* __restrict__ header, uint32_t* __restrict__ fcount) {
   __shared__ float red[6][256];
   float v[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
   for (int64_t b = threadIdx.x; b < nb; b += 256)
@@ -144,6 +187,28 @@ __global__ __launch_bounds__(256) void k_bounds_final(const float* __restrict__ 
   }
   if (threadIdx.x < 3) header[threadIdx.x] = f2o(red[threadIdx.x][0]);
   else if (threadIdx.x < 6) header[threadIdx.x] = f2o(-red[threadIdx.x][0]);
+  // fragment offsets: each thread scans a contiguous chunk of blocks
+  __shared__ uint32_t part[256];
+  const int t = threadIdx.x;
+  const int64_t chunk = (nb + 255) / 256;
+  const int64_t b0 = t * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  uint32_t sum = 0;
+  for (int64_t b = b0; b < b1; ++b) sum += fcount[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t u = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += u;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (int64_t b = b0; b < b1; ++b) {
+    const uint32_t c = fcount[b];
+    fcount[b] = run;
+    run += c;
+  }
+  if (t == 255) header[9] = part[255];
 }
 
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
@@ -163,14 +228,34 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
 // ceil(s_max / s_mid) rule; ceil(0.75 s_max / s_mid), ceil(1.25 ...), ceil(1.5 ...) and at most 3 or 6 fragments
 // all cost more in forward + build.)
 // Boxes are padded by a relative 4e-6 so they stay conservative under rounding. Each fragment gets the 27-bit
-// Morton code (9 bits per axis) of its box centre with a 3-bit box size class inserted after its first 15 bits;
-// unused slots get key 0xffffffff. The leaf count L is accumulated in header[9].
-__global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* __restrict__ header,
-                                                        float* __restrict__ aabb, uint32_t* __restrict__ fprim,
-                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  __shared__ uint32_t cnt[4];
+// Morton code (9 bits per axis) of its box centre with a 3-bit box size class inserted after its first 15 bits.
+// Fragments are written compactly in primitive order (fragment j of primitive i at foff[block] + the fragment count
+// of the block's earlier primitives + j), so the sort and the tree see only the L fragments in use.
+__global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, const uint32_t* __restrict__ header,
+                                                        const uint32_t* __restrict__ foff, float* __restrict__ aabb,
+                                                        uint32_t* __restrict__ fprim, uint32_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals) {
+  __shared__ uint32_t wsum[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int F = 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int F = 0, ax = 0;
+  float sv[3] = {0.f, 0.f, 0.f};
+  if (i < P.num_prims) {
+    sv[0] = P.scales[3 * i];
+    sv[1] = P.scales[3 * i + 1];
+    sv[2] = P.scales[3 * i + 2];
+    F = fragment_count(sv, ax);
+  }
+  // the primitive's first fragment: block offset + exclusive prefix of F over the block
+  int inc = F;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o);
+    inc += lane >= o ? u : 0;
+  }
+  if (lane == 63) wsum[wave] = (uint32_t)inc;
+  __syncthreads();
+  uint32_t first = foff[blockIdx.x] + (uint32_t)(inc - F);
+  for (int w = 0; w < wave; ++w) first += wsum[w];
   if (i < P.num_prims) {
     float lo[3] = {o2f(header[0]), o2f(header[1]), o2f(header[2])};
     float hi[3] = {o2f(header[3]), o2f(header[4]), o2f(header[5])};
@@ -180,27 +265,14 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* 
     const float Rm[3][3] = {{1.f - 2.f * (y * y + z * z), 2.f * (x * y - r * z), 2.f * (x * z + r * y)},
                             {2.f * (x * y + r * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z - r * x)},
                             {2.f * (x * z - r * y), 2.f * (y * z + r * x), 1.f - 2.f * (x * x + y * y)}};
-    const float sv[3] = {P.scales[3 * i], P.scales[3 * i + 1], P.scales[3 * i + 2]};
     const float mv[3] = {P.means[3 * i], P.means[3 * i + 1], P.means[3 * i + 2]};
-    int ax = 0;
-    if (sv[1] > sv[ax]) ax = 1;
-    if (sv[2] > sv[ax]) ax = 2;
-    float mid = 0.f;
-    for (int k = 0; k < 3; ++k)
-      if (k != ax) mid = fmaxf(mid, sv[k]);
-    F = mid > 0.f ? (int)ceilf(sv[ax] / mid) : kMaxFragments;
-    F = F < 1 ? 1 : (F > kMaxFragments ? kMaxFragments : F);
     float tight[3];
     for (int k = 0; k < 3; ++k)
       tight[k] = sqrtf(Rm[k][0] * Rm[k][0] * sv[0] * sv[0] + Rm[k][1] * Rm[k][1] * sv[1] * sv[1] +
                        Rm[k][2] * Rm[k][2] * sv[2] * sv[2]);
-    for (int j = 0; j < kMaxFragments; ++j) {
-      const int64_t slot = (int64_t)kMaxFragments * i + j;
+    for (int j = 0; j < F; ++j) {
+      const int64_t slot = (int64_t)first + j;
       vals[slot] = (uint32_t)slot;
-      if (j >= F) {
-        keys[slot] = 0xffffffffu;
-        continue;
-      }
       const float a = -1.f + 2.f * j / F, b = -1.f + 2.f * (j + 1) / F;
       float* bx = aabb + 6 * slot;
       uint32_t qk[3];
@@ -243,22 +315,15 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, uint32_t* 
       fprim[slot] = (uint32_t)i;
     }
   }
-  // leaf count: wave sums, one atomic per block
-  int v = F;
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = (uint32_t)v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t t = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-    if (t) atomicAdd(header + 9, t);
-  }
 }
 
 // ---- LSD radix sort, 8-bit digits, tiles of kSortTile keys --------------------------------------------------
 // Histogram: hist is digit-major, hist[digit * tiles + tile].
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                             uint32_t* __restrict__ hist, int64_t tiles) {
+                                                             uint32_t* __restrict__ hist, int64_t tiles,
+                                                             const uint32_t* __restrict__ dcount) {
   __shared__ uint32_t h[kRadix];
+  if (dcount) n = (int64_t)*dcount < n ? (int64_t)*dcount : n;
   h[threadIdx.x] = 0;
   __syncthreads();
   int64_t base = (int64_t)blockIdx.x * kSortTile;
@@ -298,37 +363,53 @@ __global__ __launch_bounds__(256) void k_scan_digits(uint32_t* __restrict__ hist
   if (t == 255) totals[blockIdx.x] = part[255];
 }
 
-// Stable scatter: 16 rounds of 256 consecutive keys; ranks within a wave from 8 ballots (multi-split),
-// across the 4 waves of the block from an LDS prefix.
+// Stable scatter: kSortItems rounds of 256 consecutive keys; ranks within a wave from 8 ballots (multi-split),
+// across the 4 waves of the block from an LDS prefix. The per-wave digit counts and the running digit offsets are
+// double-buffered by round parity, so a round needs two barriers: one after the counts are written, one after their
+// prefix (which also clears the other buffer and advances the offsets into it).
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout, int64_t n, int shift,
                                                                 const uint32_t* __restrict__ hist, int64_t tiles,
-                                                                const uint32_t* __restrict__ totals) {
-  __shared__ uint32_t offs[kRadix];
-  __shared__ uint32_t cnt[4][kRadix];
-  __shared__ uint32_t tot[kRadix];
+                                                                const uint32_t* __restrict__ totals,
+                                                                const uint32_t* __restrict__ dcount) {
+  __shared__ uint32_t offs[2][kRadix];
+  __shared__ uint32_t cnt[2][4][kRadix];
+  __shared__ uint32_t wsum[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // digit base = exclusive scan of the digit totals
-  tot[tid] = totals[tid];
-  __syncthreads();
-  for (int o = 1; o < kRadix; o <<= 1) {
-    const uint32_t v = tid >= o ? tot[tid - o] : 0u;
-    __syncthreads();
-    tot[tid] += v;
-    __syncthreads();
-  }
-  offs[tid] = hist[(int64_t)tid * tiles + blockIdx.x] + tot[tid] - totals[tid];
-  __syncthreads();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  if (dcount) n = (int64_t)*dcount < n ? (int64_t)*dcount : n;
+  // the tile's keys and values are loaded at once, so their latency is paid once and under the scan below
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  if (base >= n) return;
+  uint32_t key[kSortItems], val[kSortItems];
+#pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
-    int64_t idx = base + it * kSortThreads + tid;
-    bool valid = idx < n;
-    uint32_t key = valid ? kin[idx] : 0u;
-    uint32_t val = valid ? vin[idx] : 0u;
-    uint32_t digit = (key >> shift) & 0xffu;
+    const int64_t idx = base + it * kSortThreads + tid;
+    key[it] = idx < n ? kin[idx] : 0u;
+    val[it] = idx < n ? vin[idx] : 0u;
+  }
+  // digit base = exclusive scan of the digit totals (one digit per thread: wave scans, then the wave sums)
+  const uint32_t dt = totals[tid];
+  uint32_t inc = dt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o);
+    inc += lane >= o ? u : 0u;
+  }
+  if (lane == 63) wsum[wave] = inc;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) cnt[0][w][tid] = 0;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  offs[0][tid] = hist[(int64_t)tid * tiles + blockIdx.x] + before + inc - dt;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int pb = it & 1;
+    const bool valid = base + it * kSortThreads + tid < n;
+    const uint32_t digit = (key[it] >> shift) & 0xffu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -336,32 +417,26 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
       uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    uint32_t count = (uint32_t)__popcll(peers);
-    cnt[0][tid] = 0;
-    cnt[1][tid] = 0;
-    cnt[2][tid] = 0;
-    cnt[3][tid] = 0;
-    __syncthreads();
-    if (valid && rank == 0) cnt[wave][digit] = count;
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    const uint32_t count = (uint32_t)__popcll(peers);
+    if (valid && rank == 0) cnt[pb][wave][digit] = count;
     __syncthreads();
     {
-      uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid], c3 = cnt[3][tid];
-      cnt[0][tid] = 0;
-      cnt[1][tid] = c0;
-      cnt[2][tid] = c0 + c1;
-      cnt[3][tid] = c0 + c1 + c2;
-      tot[tid] = c0 + c1 + c2 + c3;
+      const uint32_t c0 = cnt[pb][0][tid], c1 = cnt[pb][1][tid], c2 = cnt[pb][2][tid], c3 = cnt[pb][3][tid];
+      cnt[pb][0][tid] = 0;
+      cnt[pb][1][tid] = c0;
+      cnt[pb][2][tid] = c0 + c1;
+      cnt[pb][3][tid] = c0 + c1 + c2;
+      offs[pb ^ 1][tid] = offs[pb][tid] + c0 + c1 + c2 + c3;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) cnt[pb ^ 1][w][tid] = 0;
     }
     __syncthreads();
     if (valid) {
-      uint32_t pos = offs[digit] + cnt[wave][digit] + rank;
-      kout[pos] = key;
-      vout[pos] = val;
+      const uint32_t pos = offs[pb][digit] + cnt[pb][wave][digit] + rank;
+      kout[pos] = key[it];
+      vout[pos] = val[it];
     }
-    __syncthreads();
-    offs[tid] += tot[tid];
-    __syncthreads();
   }
 }
 
@@ -423,6 +498,10 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(const uint32_t* __re
                                                             float* __restrict__ suf, float* __restrict__ tab0) {
   __shared__ float sh[6][kLeafBlock];
   const int64_t n = header[9];
+  if ((int64_t)blockIdx.x * kLeafBlock >= n) {  // beyond the leaves in use: an empty block box
+    if (threadIdx.x == 0) box_store(tab0 + 6 * (int64_t)blockIdx.x, box_empty());
+    return;
+  }
   const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + threadIdx.x;
   Box b = box_empty();
   if (sidx < n) {
@@ -513,13 +592,10 @@ __device__ __forceinline__ Box child_box(int32_t c, const float* sbox, const flo
   return c < 0 ? box_load(sbox + 6 * (int64_t)(~c)) : box_load(ibox + 6 * (int64_t)c);
 }
 
-__global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2, float4* __restrict__ nodes4,
-                                               const float* __restrict__ sbox, const float* __restrict__ ibox,
-                                               const uint32_t* __restrict__ header,
-                                               const uint32_t* __restrict__ order, const uint32_t* __restrict__ fprim) {
-  const int64_t n = header[9];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
+__device__ __forceinline__ void node4_record(const float4* __restrict__ nodes2, const float* __restrict__ sbox,
+                                             const float* __restrict__ ibox, const uint32_t* __restrict__ order,
+                                             const uint32_t* __restrict__ fprim, int64_t i, float4 (*stage)[257],
+                                             int t) {
   const float4 e = nodes2[i];
   // Children of the 4-wide record: start from the node's two children and open, twice, the internal slot with the
   // largest box surface (it is replaced by its two children), so a record has four children whenever its binary
@@ -574,28 +650,51 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
   };
   const int32_t ref[4] = {ref_of(r0), ref_of(r1), ref_of(r2), ref_of(r3)};
   const Box bx[4] = {b0, b1, b2, b3};
-  float4* nd = nodes4 + 8 * i;
-  nd[0] = make_float4(bx[0].lx, bx[1].lx, bx[2].lx, bx[3].lx);
-  nd[1] = make_float4(bx[0].hx, bx[1].hx, bx[2].hx, bx[3].hx);
-  nd[2] = make_float4(bx[0].ly, bx[1].ly, bx[2].ly, bx[3].ly);
-  nd[3] = make_float4(bx[0].hy, bx[1].hy, bx[2].hy, bx[3].hy);
-  nd[4] = make_float4(bx[0].lz, bx[1].lz, bx[2].lz, bx[3].lz);
-  nd[5] = make_float4(bx[0].hz, bx[1].hz, bx[2].hz, bx[3].hz);
-  nd[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
-  nd[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  stage[0][t] = make_float4(bx[0].lx, bx[1].lx, bx[2].lx, bx[3].lx);
+  stage[1][t] = make_float4(bx[0].hx, bx[1].hx, bx[2].hx, bx[3].hx);
+  stage[2][t] = make_float4(bx[0].ly, bx[1].ly, bx[2].ly, bx[3].ly);
+  stage[3][t] = make_float4(bx[0].hy, bx[1].hy, bx[2].hy, bx[3].hy);
+  stage[4][t] = make_float4(bx[0].lz, bx[1].lz, bx[2].lz, bx[3].lz);
+  stage[5][t] = make_float4(bx[0].hz, bx[1].hz, bx[2].hz, bx[3].hz);
+  stage[6][t] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
+                            __int_as_float(ref[3]));
+  stage[7][t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2, float4* __restrict__ nodes4,
+                                               const float* __restrict__ sbox, const float* __restrict__ ibox,
+                                               const uint32_t* __restrict__ header,
+                                               const uint32_t* __restrict__ order, const uint32_t* __restrict__ fprim) {
+  // the block's records are staged in LDS (one row per record float4, padded against bank conflicts) and stored
+  // as one contiguous run, so each store instruction of a wave writes 1 KiB in sequence
+  __shared__ float4 stage[8][257];
+  const int64_t n = header[9];
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  if (i0 >= n - 1) return;
+  const int64_t i = i0 + threadIdx.x;
+  if (i < n - 1) node4_record(nodes2, sbox, ibox, order, fprim, i, stage, threadIdx.x);
+  __syncthreads();
+  const int64_t nrec = (n - 1 - i0) < (int64_t)blockDim.x ? (n - 1 - i0) : (int64_t)blockDim.x;
+  float4* nd = nodes4 + 8 * i0;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int q = m * (int)blockDim.x + (int)threadIdx.x;
+    if (q < 8 * nrec) nd[q] = stage[q & 7][q >> 3];
+  }
 }
 
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
-                             int bits, hipStream_t st) {
+                             int bits, hipStream_t st, const uint32_t* dcount) {
   if (n <= 0) return hipSuccess;
   const int64_t tiles = (n + kSortTile - 1) / kSortTile;
   uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
   int passes = 0;
   for (int shift = 0; shift < bits; shift += 8, ++passes) {
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles);
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles,
+                       dcount);
     hipLaunchKernelGGL(k_scan_digits, dim3(kRadix), dim3(256), 0, st, hist, tiles, hist + (int64_t)kRadix * tiles);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
-                       shift, hist, tiles, hist + (int64_t)kRadix * tiles);
+                       shift, hist, tiles, hist + (int64_t)kRadix * tiles, dcount);
     uint32_t* t = kin; kin = kout; kout = t;
     t = vin; vin = vout; vout = t;
   }
@@ -612,7 +711,7 @@ hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   SceneLayout L = scene_layout(n);
   hipLaunchKernelGGL(k_prim_prepass<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, P, at<float4>(ws, L.rec),
-                     at<float4>(ws, L.colr), at<float>(ws, L.bpart));
+                     at<float4>(ws, L.colr), at<float>(ws, L.bpart), nullptr);
   return hipGetLastError();
 }
 
@@ -641,11 +740,14 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, header);
   if (n <= 0) return hipGetLastError();
   const unsigned nb256 = grid_for(n, 256);
-  hipLaunchKernelGGL(k_prim_prepass<false>, dim3(nb256), dim3(256), 0, st, P, rec, colr, bpart);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, bpart, (int64_t)nb256, header);
-  hipLaunchKernelGGL(k_emit_fragments, dim3(nb256), dim3(256), 0, st, P, header, aabb, fprim, k0, v0);
+  // per-block fragment counts, then offsets, live in vals1 (free until the sort, nb256 <= S words)
+  uint32_t* foff = v1;
+  hipLaunchKernelGGL(k_prim_prepass<false>, dim3(nb256), dim3(256), 0, st, P, rec, colr, bpart, foff);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, bpart, (int64_t)nb256, header, foff);
+  hipLaunchKernelGGL(k_emit_fragments, dim3(nb256), dim3(256), 0, st, P, header, foff, aabb, fprim, k0, v0);
   const int64_t S = L.slots;
-  hipError_t e = launch_sort_pairs(k0, v0, k1, v1, hist, S, 32, st);
+  // the L fragments in use (L <= S, read by the sort kernels from header[9]); keys have 30 bits
+  hipError_t e = launch_sort_pairs(k0, v0, k1, v1, hist, S, 30, st, header + 9);
   if (e != hipSuccess) return e;
   // sorted codes in keys0 (used leaves first), sorted leaf -> slot in vals0
   hipLaunchKernelGGL(k_karras, dim3(grid_for(S, 256)), dim3(256), 0, st, k0, header, nodes, range);

@@ -117,8 +117,9 @@ template <class T> inline const T* at(const void* base, size_t off) {
 
 // launchers (return hipSuccess or the first error)
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits (8 per pass); the result ends in (k0, v0).
+// With dcount, the pairs sorted are the first min(*dcount, n) (a count known only on the device); n sizes the grid.
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
-                             int bits, hipStream_t st);
+                             int bits, hipStream_t st, const uint32_t* dcount = nullptr);
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t s);
 // Rewrites the per-primitive colours of a built scene from new features (geometry and tree untouched).
 hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t s);
