@@ -363,10 +363,12 @@ __global__ __launch_bounds__(256) void k_scan_digits(uint32_t* __restrict__ hist
   if (t == 255) totals[blockIdx.x] = part[255];
 }
 
-// Stable scatter: kSortItems rounds of 256 consecutive keys; ranks within a wave from 8 ballots (multi-split),
-// across the 4 waves of the block from an LDS prefix. The per-wave digit counts and the running digit offsets are
-// double-buffered by round parity, so a round needs two barriers: one after the counts are written, one after their
-// prefix (which also clears the other buffer and advances the offsets into it).
+// Stable scatter of one tile of kSortTile keys. The tile is first ranked in LDS: kSortItems rounds of 256
+// consecutive keys, ranks within a wave from 8 ballots (multi-split), across the 4 waves from an LDS prefix, placing
+// each key at its tile-local sorted position (the tile's digit start + the keys of that digit before it). The
+// per-wave digit counts and the running digit offsets are double-buffered by round parity, so a round needs two
+// barriers. Then the sorted tile is written out in order: consecutive threads write consecutive addresses of a
+// digit's run (a few hundred bytes on average) instead of one isolated word per digit and round.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
@@ -374,41 +376,61 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
                                                                 const uint32_t* __restrict__ hist, int64_t tiles,
                                                                 const uint32_t* __restrict__ totals,
                                                                 const uint32_t* __restrict__ dcount) {
+  __shared__ uint32_t skey[kSortTile], sval[kSortTile];
+  __shared__ uint32_t lstart[kRadix], gbase[kRadix], th[kRadix];
   __shared__ uint32_t offs[2][kRadix];
   __shared__ uint32_t cnt[2][4][kRadix];
-  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t wsum[2][4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (dcount) n = (int64_t)*dcount < n ? (int64_t)*dcount : n;
-  // the tile's keys and values are loaded at once, so their latency is paid once and under the scan below
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
   if (base >= n) return;
+  const int nv = n - base < kSortTile ? (int)(n - base) : kSortTile;
+  // the tile's keys and values are loaded at once, so their latency is paid once
   uint32_t key[kSortItems], val[kSortItems];
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
-    const int64_t idx = base + it * kSortThreads + tid;
-    key[it] = idx < n ? kin[idx] : 0u;
-    val[it] = idx < n ? vin[idx] : 0u;
+    const int l = it * kSortThreads + tid;
+    key[it] = l < nv ? kin[base + l] : 0u;
+    val[it] = l < nv ? vin[base + l] : 0u;
   }
-  // digit base = exclusive scan of the digit totals (one digit per thread: wave scans, then the wave sums)
-  const uint32_t dt = totals[tid];
-  uint32_t inc = dt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(inc, o);
-    inc += lane >= o ? u : 0u;
-  }
-  if (lane == 63) wsum[wave] = inc;
+  th[tid] = 0;
 #pragma unroll
   for (int w = 0; w < 4; ++w) cnt[0][w][tid] = 0;
-  __syncthreads();
-  uint32_t before = 0;
-  for (int w = 0; w < wave; ++w) before += wsum[w];
-  offs[0][tid] = hist[(int64_t)tid * tiles + blockIdx.x] + before + inc - dt;
+  lds_barrier();
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it)
+    if (it * kSortThreads + tid < nv) atomicAdd(&th[(key[it] >> shift) & 0xffu], 1u);
+  lds_barrier();
+  // exclusive scans over the 256 digits (one per thread): the tile's digit counts give the tile-local digit starts,
+  // the digit totals the digits' bases in the output
+  const uint32_t tc = th[tid], dt = totals[tid];
+  uint32_t inc = tc, inc2 = dt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o), u2 = __shfl_up(inc2, o);
+    inc += lane >= o ? u : 0u;
+    inc2 += lane >= o ? u2 : 0u;
+  }
+  if (lane == 63) {
+    wsum[0][wave] = inc;
+    wsum[1][wave] = inc2;
+  }
+  lds_barrier();
+  uint32_t before = 0, before2 = 0;
+  for (int w = 0; w < wave; ++w) {
+    before += wsum[0][w];
+    before2 += wsum[1][w];
+  }
+  const uint32_t ls = before + inc - tc;
+  lstart[tid] = ls;
+  offs[0][tid] = ls;
+  gbase[tid] = hist[(int64_t)tid * tiles + blockIdx.x] + before2 + inc2 - dt;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
     const int pb = it & 1;
-    const bool valid = base + it * kSortThreads + tid < n;
+    const bool valid = it * kSortThreads + tid < nv;
     const uint32_t digit = (key[it] >> shift) & 0xffu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -420,7 +442,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     const uint32_t count = (uint32_t)__popcll(peers);
     if (valid && rank == 0) cnt[pb][wave][digit] = count;
-    __syncthreads();
+    lds_barrier();
     {
       const uint32_t c0 = cnt[pb][0][tid], c1 = cnt[pb][1][tid], c2 = cnt[pb][2][tid], c3 = cnt[pb][3][tid];
       cnt[pb][0][tid] = 0;
@@ -431,11 +453,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
 #pragma unroll
       for (int w = 0; w < 4; ++w) cnt[pb ^ 1][w][tid] = 0;
     }
-    __syncthreads();
+    lds_barrier();
     if (valid) {
       const uint32_t pos = offs[pb][digit] + cnt[pb][wave][digit] + rank;
-      kout[pos] = key[it];
-      vout[pos] = val[it];
+      skey[pos] = key[it];
+      sval[pos] = val[it];
+    }
+  }
+  lds_barrier();
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int l = it * kSortThreads + tid;
+    if (l < nv) {
+      const uint32_t k = skey[l];
+      const uint32_t d = (k >> shift) & 0xffu;
+      const uint32_t gp = gbase[d] + ((uint32_t)l - lstart[d]);
+      kout[gp] = k;
+      vout[gp] = sval[l];
     }
   }
 }

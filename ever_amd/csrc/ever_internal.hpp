@@ -115,6 +115,11 @@ template <class T> inline const T* at(const void* base, size_t off) {
   return reinterpret_cast<const T*>(static_cast<const char*>(base) + off);
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations and meets the other waves, without
+// draining its global stores as __syncthreads does (a loop that scatters to global memory between barriers would
+// otherwise wait out every store's latency).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // launchers (return hipSuccess or the first error)
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits (8 per pass); the result ends in (k0, v0).
 // With dcount, the pairs sorted are the first min(*dcount, n) (a count known only on the device); n sizes the grid.
