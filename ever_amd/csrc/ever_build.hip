@@ -523,54 +523,75 @@ __device__ __forceinline__ void box_store(float* p, Box b) {
   p[0] = b.lx; p[1] = b.ly; p[2] = b.lz; p[3] = b.hx; p[4] = b.hy; p[5] = b.hz;
 }
 
-// Per sorted leaf, blocks of kLeafBlock leaves: the leaf box in sorted order, in-block prefix and suffix unions,
-// and the block union (level 0 of the sparse table). Slots beyond the L used leaves are empty boxes.
+// Per sorted leaf, blocks of kLeafBlock = 64 leaves (one wave): the leaf box in sorted order, in-block prefix and
+// suffix unions (lane shuffles), and the block union (level 0 of the sparse table). Blocks beyond the L used
+// leaves get an empty block box.
+static_assert(kLeafBlock == 64, "k_leaf_blocks scans with the lanes of one wave");
 __global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(const uint32_t* __restrict__ header,
                                                             const float* __restrict__ aabb,
                                                             const uint32_t* __restrict__ order,
                                                             float* __restrict__ sbox, float* __restrict__ pre,
                                                             float* __restrict__ suf, float* __restrict__ tab0) {
-  __shared__ float sh[6][kLeafBlock];
   const int64_t n = header[9];
-  if ((int64_t)blockIdx.x * kLeafBlock >= n) {  // beyond the leaves in use: an empty block box
-    if (threadIdx.x == 0) box_store(tab0 + 6 * (int64_t)blockIdx.x, box_empty());
+  const int lane = threadIdx.x;
+  if ((int64_t)blockIdx.x * kLeafBlock >= n) {
+    if (lane == 0) box_store(tab0 + 6 * (int64_t)blockIdx.x, box_empty());
     return;
   }
-  const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + threadIdx.x;
+  const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + lane;
   Box b = box_empty();
   if (sidx < n) {
     b = box_load(aabb + 6 * (int64_t)order[sidx]);
     box_store(sbox + 6 * sidx, b);
   }
-  // inclusive prefix (Hillis-Steele) over the block in LDS
-  float v[6] = {b.lx, b.ly, b.lz, -b.hx, -b.hy, -b.hz};  // all as minima
-  for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int o = 1; o < kLeafBlock; o <<= 1) {
-    float t[6];
-    for (int k = 0; k < 6; ++k) t[k] = threadIdx.x >= o ? sh[k][threadIdx.x - o] : INFINITY;
-    __syncthreads();
-    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], t[k]);
-    __syncthreads();
+  // all six bounds as minima
+  const float v[6] = {b.lx, b.ly, b.lz, -b.hx, -b.hy, -b.hz};
+  float p[6], q[6];
+  for (int k = 0; k < 6; ++k) p[k] = q[k] = v[k];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float up = __shfl_up(p[k], o), dn = __shfl_down(q[k], o);
+      if (lane >= o) p[k] = fminf(p[k], up);
+      if (lane + o < 64) q[k] = fminf(q[k], dn);
+    }
   }
-  if (sidx < n) box_store(pre + 6 * sidx, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
-                                           -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
-  if (threadIdx.x == kLeafBlock - 1)
-    box_store(tab0 + 6 * (int64_t)blockIdx.x, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
-                                               -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
-  __syncthreads();
-  // inclusive suffix
-  for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int o = 1; o < kLeafBlock; o <<= 1) {
-    float t[6];
-    for (int k = 0; k < 6; ++k) t[k] = threadIdx.x + o < kLeafBlock ? sh[k][threadIdx.x + o] : INFINITY;
-    __syncthreads();
-    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], t[k]);
-    __syncthreads();
+  if (sidx < n) {
+    box_store(pre + 6 * sidx, {p[0], p[1], p[2], -p[3], -p[4], -p[5]});
+    box_store(suf + 6 * sidx, {q[0], q[1], q[2], -q[3], -q[4], -q[5]});
   }
-  if (sidx < n) box_store(suf + 6 * sidx, {sh[0][threadIdx.x], sh[1][threadIdx.x], sh[2][threadIdx.x],
-                                           -sh[3][threadIdx.x], -sh[4][threadIdx.x], -sh[5][threadIdx.x]});
+  if (lane == 63) box_store(tab0 + 6 * (int64_t)blockIdx.x, {p[0], p[1], p[2], -p[3], -p[4], -p[5]});
+}
+
+// Sparse table levels 1..min(levels - 1, kTabFused) in one launch: a block loads its 256 entries of level 0 and the
+// 2^kTabFused - 1 after them into LDS (entries beyond nb as empty boxes) and builds the levels there, each one
+// shorter by its half-width, storing its own 256 entries of every level.
+constexpr int kTabFused = 8;
+__global__ __launch_bounds__(256) void k_table_levels_low(float* __restrict__ tab, int64_t nb, int levels) {
+  constexpr int W = 256 + (1 << kTabFused) - 1;
+  __shared__ float lds[2][6][W];
+  const int tid = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * 256;
+  for (int j = tid; j < W; j += 256) {
+    const Box x = b0 + j < nb ? box_load(tab + 6 * (b0 + j)) : box_empty();
+    lds[0][0][j] = x.lx; lds[0][1][j] = x.ly; lds[0][2][j] = x.lz;
+    lds[0][3][j] = -x.hx; lds[0][4][j] = -x.hy; lds[0][5][j] = -x.hz;
+  }
+  __syncthreads();
+  const int top = levels - 1 < kTabFused ? levels - 1 : kTabFused;
+  int valid = W;
+  for (int k = 1; k <= top; ++k) {
+    const int half = 1 << (k - 1), sb = (k - 1) & 1, db = k & 1;
+    valid -= half;
+    for (int j = tid; j < valid; j += 256)
+      for (int f = 0; f < 6; ++f) lds[db][f][j] = fminf(lds[sb][f][j], lds[sb][f][j + half]);
+    __syncthreads();
+    if (b0 + tid < nb)
+      box_store(tab + (int64_t)k * 6 * nb + 6 * (b0 + tid),
+                {lds[db][0][tid], lds[db][1][tid], lds[db][2][tid], -lds[db][3][tid], -lds[db][4][tid],
+                 -lds[db][5][tid]});
+  }
 }
 
 // Sparse table level k over block unions: tab_k[b] = tab_{k-1}[b] U tab_{k-1}[b + 2^(k-1)].
@@ -788,7 +809,9 @@ hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t st) {
   const int64_t nb = L.leaf_blocks;
   hipLaunchKernelGGL(k_leaf_blocks, dim3((unsigned)nb), dim3(kLeafBlock), 0, st, header, aabb, v0, sbox, pre, suf,
                      tab);
-  for (int k = 1; k < L.tab_levels; ++k)
+  if (L.tab_levels > 1)
+    hipLaunchKernelGGL(k_table_levels_low, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab, nb, L.tab_levels);
+  for (int k = kTabFused + 1; k < L.tab_levels; ++k)
     hipLaunchKernelGGL(k_table_level, dim3(grid_for(nb, 256)), dim3(256), 0, st, tab + (int64_t)(k - 1) * 6 * nb,
                        tab + (int64_t)k * 6 * nb, nb, (int64_t)1 << (k - 1));
   // the fragment boxes (aabb) are dead once the leaf blocks are built: their region holds the binary-node boxes
