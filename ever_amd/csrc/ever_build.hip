@@ -739,12 +739,12 @@ __global__ __launch_bounds__(256) void k_node4(const float4* __restrict__ nodes2
 }
 
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
-                             int bits, hipStream_t st, const uint32_t* dcount) {
+                             int bits, hipStream_t st, const uint32_t* dcount, int lo_bit, uint32_t** vres) {
   if (n <= 0) return hipSuccess;
   const int64_t tiles = (n + kSortTile - 1) / kSortTile;
   uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
   int passes = 0;
-  for (int shift = 0; shift < bits; shift += 8, ++passes) {
+  for (int shift = lo_bit; shift < bits; shift += 8, ++passes) {
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, n, shift, hist, tiles,
                        dcount);
     hipLaunchKernelGGL(k_scan_digits, dim3(kRadix), dim3(256), 0, st, hist, tiles, hist + (int64_t)kRadix * tiles);
@@ -752,6 +752,10 @@ hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
                        shift, hist, tiles, hist + (int64_t)kRadix * tiles, dcount);
     uint32_t* t = kin; kin = kout; kout = t;
     t = vin; vin = vout; vout = t;
+  }
+  if (vres) {
+    *vres = vin;
+    return hipGetLastError();
   }
   if (passes & 1) {  // bring the result back to (k0, v0)
     hipError_t e = hipMemcpyAsync(k0, kin, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);

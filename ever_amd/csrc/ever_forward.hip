@@ -1250,8 +1250,12 @@ hipError_t launch_forward(const ever_prims& P, void* ws, const ever_rays& R, con
     hipLaunchKernelGGL(k_ray_keys, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, R.origins, R.directions,
                        R.num_rays, R.tmin, header, R.view_index, P.view_origins, views ? P.num_views : 0, vfirst, k0,
                        v0);
+    // the low 6 bits order rays within an 8 x 8 tile (or a cell of 4 x 4 x 4) and are left unsorted (three passes
+    // instead of four): a wave takes 64 consecutive rays, and a tile keeps its input order, which moves rays between
+    // waves only where a tile straddles two (at 1920 x 1080 a wave's pixels then span 64.9 pixels of bounding box on
+    // average, against 64.8 with the full sort)
     hipError_t e = launch_sort_pairs(k0, v0, at<uint32_t>(O.workspace, FL.keys1), at<uint32_t>(O.workspace, FL.vals1),
-                                     at<uint32_t>(O.workspace, FL.hist), R.num_rays, 30, st);
+                                     at<uint32_t>(O.workspace, FL.hist), R.num_rays, 30, st, nullptr, 6, &v0);
     if (e != hipSuccess) return e;
   } else {
     hipLaunchKernelGGL(k_iota, dim3(grid_for(R.num_rays, 256)), dim3(256), 0, st, v0, R.num_rays);

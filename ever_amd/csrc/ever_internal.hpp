@@ -121,10 +121,12 @@ template <class T> inline const T* at(const void* base, size_t off) {
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // launchers (return hipSuccess or the first error)
-// Stable LSD radix sort of (key, value) pairs on the low `bits` bits (8 per pass); the result ends in (k0, v0).
+// Stable LSD radix sort of (key, value) pairs on key bits [lo_bit, bits) (8 per pass); the result ends in (k0, v0),
+// or, when vres is given, in whichever pair the last pass wrote, with *vres set to its value buffer (no copy back).
 // With dcount, the pairs sorted are the first min(*dcount, n) (a count known only on the device); n sizes the grid.
 hipError_t launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int64_t n,
-                             int bits, hipStream_t st, const uint32_t* dcount = nullptr);
+                             int bits, hipStream_t st, const uint32_t* dcount = nullptr, int lo_bit = 0,
+                             uint32_t** vres = nullptr);
 hipError_t launch_build(const ever_prims& P, void* ws, hipStream_t s);
 // Rewrites the per-primitive colours of a built scene from new features (geometry and tree untouched).
 hipError_t launch_update_colors(const ever_prims& P, void* ws, hipStream_t s);
