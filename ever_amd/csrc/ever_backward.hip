@@ -444,16 +444,17 @@ struct EpiArgs {
   float* vgrad;
 };
 
-// dL/df[p,k,c] = Y_k(dir_p) * g[p,c], one lane per output element so the (N, d, 3) writes are coalesced.
+// dL/df[p,k,c] = Y_k(dir_p) * g[p,c], one lane per primitive: the basis once, then its d x 3 gradients
+// (coefficients beyond the degree are zero). With d = 16 the block's 256 x 192 B of gradients are staged in LDS
+// and stored as one contiguous run.
 __global__ __launch_bounds__(256) void k_sh_feature_grads(const float* __restrict__ grgb,
                                                           const float* __restrict__ means, float ox, float oy,
                                                           float oz, int64_t n, int d, int deg,
                                                           float* __restrict__ dfeat) {
-  // one lane per primitive: the basis once, then its d x 3 gradients (coefficients beyond the degree are zero)
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x, p = p0 + threadIdx.x;
+  const bool valid = p < n;
   const int nc = (deg + 1) * (deg + 1);
-  const float4 g = reinterpret_cast<const float4*>(grgb)[p];
+  const float4 g = valid ? reinterpret_cast<const float4*>(grgb)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   float bas[16];
   if (g.x != 0.f || g.y != 0.f || g.z != 0.f) {
     const f3 dir = sh_dir(ld3(means + 3 * p), mk3(ox, oy, oz));
@@ -462,28 +463,40 @@ __global__ __launch_bounds__(256) void k_sh_feature_grads(const float* __restric
 #pragma unroll
     for (int t = 0; t < 16; ++t) bas[t] = 0.f;
   }
+  if (d == 16 && (reinterpret_cast<uintptr_t>(dfeat) & 15u) == 0) {
+    __shared__ float4 st[256 * 13];  // 12 float4 per primitive, rows padded to 13 against bank conflicts
+    if (valid) {
+      float v[48];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const float b = t < nc ? bas[t] : 0.f;
+        v[3 * t] = b * g.x;
+        v[3 * t + 1] = b * g.y;
+        v[3 * t + 2] = b * g.z;
+      }
+#pragma unroll
+      for (int q = 0; q < 12; ++q)
+        st[threadIdx.x * 13 + q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    }
+    __syncthreads();
+    const int nblk = n - p0 < 256 ? (int)(n - p0) : 256;
+    float4* out4 = reinterpret_cast<float4*>(dfeat + 48 * p0);
+#pragma unroll
+    for (int it = 0; it < 12; ++it) {
+      const int q = it * 256 + (int)threadIdx.x;
+      if (q < 12 * nblk) out4[q] = st[(q / 12) * 13 + q % 12];
+    }
+    return;
+  }
+  if (!valid) return;
   float* out = dfeat + 3 * (int64_t)d * p;
-  if (d == 16 && (reinterpret_cast<uintptr_t>(dfeat) & 15u) == 0) {  // 192 B per primitive: twelve vector stores
-    float v[48];
+  for (int t = 0; t < d; ++t) {
+    float b = 0.f;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const float b = t < nc ? bas[t] : 0.f;
-      v[3 * t] = b * g.x;
-      v[3 * t + 1] = b * g.y;
-      v[3 * t + 2] = b * g.z;
-    }
-#pragma unroll
-    for (int q = 0; q < 12; ++q)
-      reinterpret_cast<float4*>(out)[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-  } else {
-    for (int t = 0; t < d; ++t) {
-      float b = 0.f;
-#pragma unroll
-      for (int u = 0; u < 16; ++u) b = (u == t && t < nc) ? bas[u] : b;
-      out[3 * t] = b * g.x;
-      out[3 * t + 1] = b * g.y;
-      out[3 * t + 2] = b * g.z;
-    }
+    for (int u = 0; u < 16; ++u) b = (u == t && t < nc) ? bas[u] : b;
+    out[3 * t] = b * g.x;
+    out[3 * t + 1] = b * g.y;
+    out[3 * t + 2] = b * g.z;
   }
 }
 

@@ -235,7 +235,10 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, const uint
                                                         const uint32_t* __restrict__ foff, float* __restrict__ aabb,
                                                         uint32_t* __restrict__ fprim, uint32_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals) {
+  // the block's fragments (at most 4 per primitive) are staged in LDS and stored as one contiguous run
   __shared__ uint32_t wsum[4];
+  __shared__ float sbox[6 * kMaxFragments * 256];
+  __shared__ uint32_t skey[kMaxFragments * 256], sprim[kMaxFragments * 256];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int F = 0, ax = 0;
@@ -254,8 +257,10 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, const uint
   }
   if (lane == 63) wsum[wave] = (uint32_t)inc;
   __syncthreads();
-  uint32_t first = foff[blockIdx.x] + (uint32_t)(inc - F);
+  const uint32_t fb = foff[blockIdx.x];
+  uint32_t first = (uint32_t)(inc - F);  // within the block
   for (int w = 0; w < wave; ++w) first += wsum[w];
+  const int total = (int)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
   if (i < P.num_prims) {
     float lo[3] = {o2f(header[0]), o2f(header[1]), o2f(header[2])};
     float hi[3] = {o2f(header[3]), o2f(header[4]), o2f(header[5])};
@@ -271,10 +276,9 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, const uint
       tight[k] = sqrtf(Rm[k][0] * Rm[k][0] * sv[0] * sv[0] + Rm[k][1] * Rm[k][1] * sv[1] * sv[1] +
                        Rm[k][2] * Rm[k][2] * sv[2] * sv[2]);
     for (int j = 0; j < F; ++j) {
-      const int64_t slot = (int64_t)first + j;
-      vals[slot] = (uint32_t)slot;
+      const int slot = (int)first + j;
       const float a = -1.f + 2.f * j / F, b = -1.f + 2.f * (j + 1) / F;
-      float* bx = aabb + 6 * slot;
+      float* bx = sbox + 6 * slot;
       uint32_t qk[3];
       float maxext = 0.f, sext = 0.f;
       for (int k = 0; k < 3; ++k) {
@@ -311,9 +315,16 @@ __global__ __launch_bounds__(256) void k_emit_fragments(ever_prims P, const uint
       const float rel = sext > 0.f && maxext > 0.f ? sext / maxext : 256.f;
       const uint32_t sc = (uint32_t)fminf(fmaxf(floorf(log2f(rel)) - 1.f, 0.f), 7.f);
       const uint32_t m27 = (expand_bits10(qk[0]) << 2) | (expand_bits10(qk[1]) << 1) | expand_bits10(qk[2]);
-      keys[slot] = ((m27 >> 12) << 15) | (sc << 12) | (m27 & 0xfffu);
-      fprim[slot] = (uint32_t)i;
+      skey[slot] = ((m27 >> 12) << 15) | (sc << 12) | (m27 & 0xfffu);
+      sprim[slot] = (uint32_t)i;
     }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < 6 * total; l += 256) aabb[6 * (int64_t)fb + l] = sbox[l];
+  for (int l = threadIdx.x; l < total; l += 256) {
+    keys[fb + l] = skey[l];
+    fprim[fb + l] = sprim[l];
+    vals[fb + l] = fb + (uint32_t)l;
   }
 }
 
