@@ -549,12 +549,12 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(const uint32_t* __re
     if (lane == 0) box_store(tab0 + 6 * (int64_t)blockIdx.x, box_empty());
     return;
   }
-  const int64_t sidx = (int64_t)blockIdx.x * kLeafBlock + lane;
+  const int64_t s0 = (int64_t)blockIdx.x * kLeafBlock, sidx = s0 + lane;
+  // the block's leaf, prefix and suffix boxes are staged in LDS and stored as contiguous runs
+  __shared__ float st[3][6 * kLeafBlock];
   Box b = box_empty();
-  if (sidx < n) {
-    b = box_load(aabb + 6 * (int64_t)order[sidx]);
-    box_store(sbox + 6 * sidx, b);
-  }
+  if (sidx < n) b = box_load(aabb + 6 * (int64_t)order[sidx]);
+  box_store(&st[0][6 * lane], b);
   // all six bounds as minima
   const float v[6] = {b.lx, b.ly, b.lz, -b.hx, -b.hy, -b.hz};
   float p[6], q[6];
@@ -568,11 +568,16 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_blocks(const uint32_t* __re
       if (lane + o < 64) q[k] = fminf(q[k], dn);
     }
   }
-  if (sidx < n) {
-    box_store(pre + 6 * sidx, {p[0], p[1], p[2], -p[3], -p[4], -p[5]});
-    box_store(suf + 6 * sidx, {q[0], q[1], q[2], -q[3], -q[4], -q[5]});
-  }
+  box_store(&st[1][6 * lane], {p[0], p[1], p[2], -p[3], -p[4], -p[5]});
+  box_store(&st[2][6 * lane], {q[0], q[1], q[2], -q[3], -q[4], -q[5]});
   if (lane == 63) box_store(tab0 + 6 * (int64_t)blockIdx.x, {p[0], p[1], p[2], -p[3], -p[4], -p[5]});
+  __syncthreads();
+  const int nv = n - s0 < kLeafBlock ? (int)(n - s0) : kLeafBlock;
+  for (int l = lane; l < 6 * nv; l += kLeafBlock) {
+    sbox[6 * s0 + l] = st[0][l];
+    pre[6 * s0 + l] = st[1][l];
+    suf[6 * s0 + l] = st[2][l];
+  }
 }
 
 // Sparse table levels 1..min(levels - 1, kTabFused) in one launch: a block loads its 256 entries of level 0 and the
